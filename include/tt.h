@@ -1,0 +1,197 @@
+/*
+ * tt.h -- C ABI of libtt.so, the MI355X (gfx950) kernels behind the
+ * turbulence_tracing drop-in.
+ *
+ * The reference (jdhare/turbulence_tracing) is pure Python and has no FFI;
+ * each entry point below replaces the numpy/scipy work inside one reference
+ * function, cited as file:line relative to the reference root.  The Python
+ * modules of turbulence_tracing_amd bind these with ctypes.
+ *
+ * Conventions
+ *  - Every pointer argument named as device memory must be device memory
+ *    (hipMalloc / torch CUDA tensor); structs are host memory.
+ *  - Every call is asynchronous on `stream` (a hipStream_t, NULL = the
+ *    default stream) and returns TT_OK or an error code; tt_last_error()
+ *    returns a thread-local message for the last failure.
+ *  - Ray arrays are SoA rows of length ld (>= n): s0/sf are 6 x ld float64
+ *    [x, y, z, vx, vy, vz] in m and m/s; rf is 4 x ld float64
+ *    [x, theta, y, phi] in m and rad (mm after the optics, like the
+ *    reference's m_to_mm).
+ *  - Grid fields are stored x-fastest, z-slowest: index (iz*ny + iy)*nx + ix.
+ *  - The caller owns every device buffer.  The library allocates only inside
+ *    an FFT plan (its twiddle table), released by tt_fft_plan_destroy.
+ */
+#ifndef TT_H
+#define TT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TT_OK 0
+#define TT_ERR_ARG 1
+#define TT_ERR_HIP 2
+#define TT_ERR_UNSUPPORTED 3
+
+int tt_version(void);
+const char *tt_last_error(void);
+
+/* Uniform grid: node (ix, iy, iz) sits at (x0 + ix dx, y0 + iy dy, z0 + iz dz).
+ * Mirrors the x, y, z coordinate arrays of ElectronCube
+ * (particle_tracking/particle_tracker.py:125-145). */
+typedef struct {
+    int32_t nx, ny, nz;
+    double x0, y0, z0;
+    double dx, dy, dz;
+} tt_grid;
+
+/* ------------------------------------------------------------------ field
+ * Replaces gaussian3D_FFT (gaussian_fields/turboGen.py:488-538): centred-grid
+ * Gaussian noise, Hermitian symmetrisation, sqrt(P(k)) shaping, zero mode,
+ * inverse 3-D FFT, real part.  Output `out` is float32 [nz][ny][nx] and holds
+ * the reference's array f[ix][iy][iz] (axis 0 = x).  When `sumsq` (device,
+ * one double) is non-NULL the sum of squares of the output is ADDED to it.
+ * Any nx, ny, nz >= 2 and <= 4096 is accepted (mixed radix 2/3/4/5/8 plus
+ * direct odd-prime passes).
+ */
+typedef struct tt_fft_plan_s *tt_fft_plan;
+int tt_fft_plan_create(int32_t nx, int32_t ny, int32_t nz, tt_fft_plan *plan);
+int tt_fft_plan_destroy(tt_fft_plan plan);
+/* bytes of device workspace the generate calls need (the half spectrum) */
+int tt_fft_workspace_bytes(tt_fft_plan plan, size_t *bytes);
+
+/* Power-law spectrum P = amp * Ka^beta on kmin <= Ka <= kmax, zero elsewhere,
+ * Ka = sqrt(kx^2 + ky^2 + (kz/aniso)^2) in cycles/sample (the units of
+ * np.fft.fftfreq used by turboGen.py:516). */
+typedef struct {
+    double beta, aniso, kmin, kmax, amp;
+    uint64_t seed;
+} tt_spectrum;
+
+/* Native generator: Philox4x32-10 noise drawn in-kernel. */
+int tt_field_generate(tt_fft_plan plan, const tt_spectrum *spec, void *workspace, float *out,
+                      double *sumsq, void *stream);
+/* Drop-in generator: the caller's centred-grid noise wr, wi and sqrt(P)
+ * (float32 device arrays, C order [cx][cy][cz], as turboGen.py:522-530 builds
+ * them on the host). */
+int tt_field_from_noise(tt_fft_plan plan, const float *wr, const float *wi, const float *sqrt_psd,
+                        void *workspace, float *out, double *sumsq, void *stream);
+
+/* ---------------------------------------------------------------- gradient
+ * Replaces calc_dndr (particle_tracking/particle_tracker.py:220-241):
+ * ne/nc = alpha + beta * f, clipped at ne_max, then np.gradient semantics
+ * (2nd-order central interior, 1st-order one-sided edges) along each axis.
+ * `field` is float32 (dtype 0) or float64 (dtype 1) with element strides
+ * (sx, sy, sz).  Writes float4 out4[iz][iy][ix] =
+ *   { -0.5 dz d(ne/nc)/dx, -0.5 dz d(ne/nc)/dy, -0.5 dz d(ne/nc)/dz,
+ *     k0 dz (sqrt(1 - ne/nc) - 1) }
+ * i.e. the reference's dndx/dndy/dndz divided by c^2 and multiplied by dz,
+ * plus the phase rate per z-cell.  When sumsq (device, one double, e.g. from
+ * tt_field_generate) is non-NULL, beta is divided by sqrt(*sumsq / count):
+ * the caller-side unit-RMS normalisation of the field, applied on the device. */
+int tt_grad_stencil(const void *field, int32_t dtype, int64_t sx, int64_t sy, int64_t sz,
+                    const tt_grid *grid, double alpha, double beta, double ne_max, double k0,
+                    const double *sumsq, int64_t count, float *out4, void *stream);
+
+/* ------------------------------------------------------------------ tracer
+ * Replaces solve + ray_at_exit + the dndr/dsdt calls inside solve_ivp
+ * (particle_tracking/particle_tracker.py:243-256, 312-380, 398-419).
+ * Fixed-step RK4 with z (in cells) as the independent variable, `h` = 1/m
+ * cells per step, from each ray's start plane to the last grid plane; the
+ * interferometric phase k0 int (n-1) ds is accumulated in float64.
+ * order (nullable, int32 device) lists the rays to process in that order
+ * (e.g. from tt_ray_sort); outputs are written at each ray's own index.
+ * sf (nullable) receives the state at c t = cT, continued straight past the
+ * exit plane (the field is zero outside the grid, PT:239-241).
+ * status (nullable, int32): 0 ok, 1 ray not moving forward in z, 2 NaN. */
+typedef struct {
+    double h;
+    double cT;
+} tt_trace_opts;
+int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *opts,
+                 const double *s0, int64_t ld_s0, int64_t n, const int32_t *order,
+                 double *rf, int64_t ld_rf, double *phase, double *sf, int64_t ld_sf,
+                 int32_t *status, void *stream);
+
+/* Trilinear samples of the node field at n points pts (3 x ld float64, m):
+ * out (4 x ld_out float64) = the four field channels, zero strictly outside
+ * the grid and NaN for NaN points -- the RegularGridInterpolator semantics of
+ * ElectronCube.dndr (particle_tracking/particle_tracker.py:239-256). */
+int tt_sample_field(const float *field4, const tt_grid *grid, const double *pts, int64_t ld, int64_t n,
+                    double *out, int64_t ld_out, void *stream);
+
+/* ------------------------------------------------------------------- beam
+ * Replaces init_beam (particle_tracking/particle_tracker.py:258-310): same
+ * distribution (position uniform in a disc of radius beam_size, polar angle
+ * chi ~ N(0, divergence), azimuth phi ~ U[0, pi)), drawn from Philox4x32-10
+ * with counter = ray index + offset.  axis: 0 = x, 1 = y, 2 = z. */
+int tt_beam_sample(int64_t n, double beam_size, double divergence, double extent, int32_t axis,
+                   uint64_t seed, uint64_t offset, double *s0, int64_t ld, void *stream);
+
+/* Ray ordering for coalesced field gathers: a counting sort of the rays by
+ * the Morton code of their entry column (cell >> brick_shift).  `scratch`
+ * must hold tt_ray_sort_scratch_bytes bytes of device memory. */
+int tt_ray_sort_scratch_bytes(const tt_grid *grid, int32_t brick_shift, size_t *bytes);
+int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
+                void *scratch, int32_t *order, void *stream);
+
+/* ------------------------------------------------------------- detectors
+ * Replaces the ray_transfer_matrix optics and Rays.histogram
+ * (particle_tracking/ray_transfer_matrix.py:37-154, 173-191, 208-299).
+ * An optical train is a list of ops applied in order to [x, theta, y, phi]
+ * in mm/rad; a rejected ray becomes NaN (RTM:77-78). */
+enum {
+    TT_OP_DISTANCE = 1, /* a = d                          RTM:62-71   */
+    TT_OP_LENS = 2,     /* a = f1, b = f2                 RTM:42-60   */
+    TT_OP_APERTURE = 3, /* reject r^2 > a^2               RTM:73-79   */
+    TT_OP_STOP = 4,     /* reject r^2 < a^2               RTM:81-87   */
+    TT_OP_KNIFE = 5,    /* axis 0=x 2=y, a = offset, b = direction sign RTM:138-154 */
+    TT_OP_RECT = 6,     /* reject x^2 > a^2 and y^2 > b^2 RTM:128-136 */
+    TT_OP_ANNULUS = 7   /* reject a^2 < r^2 < b^2          RTM:89-111  */
+};
+#define TT_MAX_OPS 24
+typedef struct {
+    int32_t op, axis;
+    double a, b;
+} tt_optic;
+typedef struct {
+    int32_t n_ops;
+    int32_t to_mm; /* 1: scale rows 0 and 2 by 1e3 first (m_to_mm, RTM:37-40) */
+    tt_optic ops[TT_MAX_OPS];
+} tt_train;
+
+/* np.histogram2d over [xmin, xmax] x [ymin, ymax] with nbx x nby bins,
+ * last bin closed, NaN dropped, stored transposed H[iy][ix] (RTM:188-191). */
+typedef struct {
+    int32_t nbx, nby;
+    double xmin, xmax, ymin, ymax;
+} tt_hist;
+
+/* Apply a train to n rays: r_out may alias r_in. */
+int tt_optics_apply(const tt_train *train, const double *r_in, int64_t ld_in, int64_t n, double *r_out,
+                    int64_t ld_out, void *stream);
+
+/* One detector of the fused detector pass.  mode 0: counts (uint32);
+ * mode 1: interferogram, weight 1 + cos(phase - tilt_k * x_det) with x_det in
+ * mm and tilt_k in rad/mm, summed as float32. */
+typedef struct {
+    tt_train train;
+    tt_hist hist;
+    int32_t mode;
+    double tilt_k;
+    uint32_t *counts; /* device, nby*nbx, mode 0 */
+    float *wsum;      /* device, nby*nbx, mode 1 */
+} tt_detector;
+#define TT_MAX_DETECTORS 4
+/* Runs up to TT_MAX_DETECTORS detectors over the same rays in one pass and
+ * ADDS into their histograms.  phase may be NULL when no detector uses it. */
+int tt_detect(const tt_detector *dets, int32_t ndet, const double *rf, int64_t ld, const double *phase,
+              int64_t n, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TT_H */

@@ -1,0 +1,98 @@
+"""CPU baseline harness: the reference's CPU path, timed on host cores.
+
+TEST INFRASTRUCTURE ONLY (see ``oracle/__init__``): used by bench.py's
+cpu_baseline leg.
+
+Pattern of particle_tracking/example_multiprocess.py:41-62 (a Pool of P
+workers, each solving its own bundle of rays) against the real API of the
+library: each worker builds the three RegularGridInterpolators of
+calc_dndr (PT:239-241) over the shared float64 gradient arrays and runs the
+as-shipped solve (PT:312-331: one solve_ivp RK45 over the bundle at scipy's
+default tolerances) followed by ray_at_exit.
+
+The gradient arrays live in POSIX shared memory, written by the parent, so
+workers (forked before the parent touches the GPU) attach by name.
+"""
+import os
+import time
+from multiprocessing import get_context, shared_memory
+
+import numpy as np
+
+_G = {}
+
+
+def _worker_init():
+    for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ[v] = "1"
+
+
+def _attach(name, shape):
+    if _G.get("name") == name:
+        return _G["cube"]
+    from . import tracer as T
+    shm = shared_memory.SharedMemory(name=name)
+    try:  # the parent owns the segment; keep this process's tracker off it
+        from multiprocessing import resource_tracker
+        resource_tracker.unregister(shm._name, "shared_memory")
+    except Exception:
+        pass
+    arr = np.ndarray((3,) + tuple(shape[:3]), dtype=np.float64, buffer=shm.buf)
+    axes = shape[3]
+    cube = T.Cube(axes[0], axes[1], axes[2])
+    from scipy.interpolate import RegularGridInterpolator
+    g = (cube.x, cube.y, cube.z)
+    cube.dndx_interp = RegularGridInterpolator(g, arr[0], bounds_error=False, fill_value=0.0)
+    cube.dndy_interp = RegularGridInterpolator(g, arr[1], bounds_error=False, fill_value=0.0)
+    cube.dndz_interp = RegularGridInterpolator(g, arr[2], bounds_error=False, fill_value=0.0)
+    cube.extent = cube.extent_z
+    _G.update(name=name, shm=shm, cube=cube)
+    return cube
+
+
+def _solve_bundle(args):
+    name, shape, s0 = args
+    cube = _attach(name, shape)
+    cube.s0 = s0
+    t = time.perf_counter()
+    import contextlib
+    import io
+    with contextlib.redirect_stdout(io.StringIO()):
+        rf = cube.solve()
+    return rf, time.perf_counter() - t, int(cube.sol.nfev)
+
+
+class CPUBaseline:
+    """Create before the process initialises the GPU (the pool forks)."""
+
+    def __init__(self, processes=None):
+        self.P = processes or min(16, os.cpu_count() or 1)
+        self.pool = get_context("fork").Pool(self.P, initializer=_worker_init)
+        self.shm = None
+
+    def load_gradients(self, dndx, dndy, dndz, x, y, z):
+        """float64 (nx, ny, nz) gradient arrays in m/s^2 (PT:235-237)."""
+        shape = dndx.shape
+        nbytes = 3 * dndx.size * 8
+        self.shm = shared_memory.SharedMemory(create=True, size=nbytes)
+        arr = np.ndarray((3,) + shape, dtype=np.float64, buffer=self.shm.buf)
+        arr[0], arr[1], arr[2] = dndx, dndy, dndz
+        self.shape = tuple(shape) + ((np.asarray(x), np.asarray(y), np.asarray(z)),)
+
+    def run(self, s0, bundles):
+        """Solve the rays s0 (6 x N) split into ``bundles`` bundles over the
+        pool.  Returns rf (4 x N), wall seconds, total nfev."""
+        parts = np.array_split(s0, bundles, axis=1)
+        t = time.perf_counter()
+        out = self.pool.map(_solve_bundle, [(self.shm.name, self.shape, p) for p in parts], chunksize=1)
+        wall = time.perf_counter() - t
+        rf = np.concatenate([o[0] for o in out], axis=1)
+        return rf, wall, sum(o[2] for o in out)
+
+    def close(self):
+        self.pool.close()
+        self.pool.join()
+        if self.shm is not None:
+            self.shm.close()
+            self.shm.unlink()
+            self.shm = None

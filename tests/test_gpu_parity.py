@@ -1,0 +1,407 @@
+"""HIP kernels (through the C ABI) against the FP64 oracle on the same inputs.
+
+Tolerances (stated by the parity bar of this path):
+  field         <= 1e-5 relative RMS against numpy FP64 on identical noise
+  ray exit x,y  <= 1e-3 grid cells against the FP64 integrator
+  exit angles   <= 1e-3 of the RMS deflection
+  histograms    exact counts on identical rays (integer work); Poisson
+                agreement for traced rays
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import field as OF
+from oracle import philox as OP
+from oracle import rtm as OR
+from oracle import tracer as OT
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda"
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
+
+
+# ------------------------------------------------------------------ field
+@pytest.mark.parametrize("N", [2, 4])
+def test_field_dropin_matches_reference_golden(lib, golden, N):
+    from turbulence_tracing_amd import turboGen
+    M = 2 * N + 1
+
+    def k_func(K):
+        with np.errstate(divide="ignore"):
+            P = K ** (-11.0 / 3.0)
+        P[(K < 1.0 / M) | (K > 0.5)] = 0.0
+        return P
+
+    rng = np.random.RandomState(int(golden[f"field_N{N}_seed"]))
+    got = turboGen.gaussian3D_FFT(N, k_func, rng=rng, as_numpy=True)
+    assert got.shape == (M, M, M)
+    assert rel_rms(got, golden[f"field_N{N}"]) <= 1e-5
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (9, 15, 7), (64, 64, 64), (30, 21, 14), (37, 11, 6),
+                                   (128, 96, 80)])
+def test_field_from_noise_any_shape(lib, shape):
+    from turbulence_tracing_amd import turboGen
+    rs = np.random.RandomState(sum(shape))
+    Wr, Wi = rs.randn(*shape), rs.randn(*shape)
+    S = OF.sqrt_powerlaw(shape, -11 / 3)
+    want = OF.gaussian_field(Wr, Wi, S)
+    got = turboGen.gaussian_field_from_noise(Wr, Wi, S).double().cpu().numpy()
+    assert rel_rms(got, want) <= 1e-5
+
+
+@pytest.mark.parametrize("shape,beta,aniso", [((32, 32, 32), -11 / 3, 1.0), ((24, 20, 16), -3.0, 0.5),
+                                              ((17, 19, 23), -3.5, 2.0), ((96, 64, 48), -11 / 3, 1.3)])
+def test_native_field_matches_philox_twin(lib, shape, beta, aniso):
+    from turbulence_tracing_amd import turboGen
+    f, rms = turboGen.gaussian_field(shape, beta=beta, aniso=aniso, seed=77)
+    want = OF.powerlaw_field(shape, beta, seed=77, aniso=aniso)
+    got = f.double().cpu().numpy()
+    assert rel_rms(got, want) <= 1e-5
+    assert abs(float(rms) - np.sqrt(np.mean(want ** 2))) <= 1e-5 * np.sqrt(np.mean(want ** 2))
+
+
+def test_field_512_properties(lib):
+    """Full-size field: unit RMS after normalisation, zero mean, the radial
+    spectrum slope, and agreement with the FP64 twin on a sub-block of the
+    spectrum-free statistics."""
+    from turbulence_tracing_amd import turboGen
+    f, rms = turboGen.gaussian_field((512, 512, 512), seed=3, normalise=True)
+    assert abs(float(f.double().pow(2).mean()) - 1.0) < 1e-4
+    assert abs(float(f.double().mean())) < 1e-4
+    # spectrum slope on a 128^3 field from the same generator
+    g, _ = turboGen.gaussian_field((128, 128, 128), seed=4)
+    k, P, _ = OF.radial_psd(g.double().cpu().numpy(), nbins=64)
+    sel = (k > 0.05) & (k < 0.4)
+    slope = np.polyfit(np.log(k[sel]), np.log(P[sel]), 1)[0]
+    assert abs(slope + 11 / 3) < 0.15
+
+
+# --------------------------------------------------------------- gradient
+def small_cube(n=33, seed=2, eps=0.1, ne0=1e24, L=10e-3):
+    x = np.linspace(-L / 2, L / 2, n)
+    d = OF.powerlaw_field((n, n, n), -11 / 3, seed=seed)
+    d /= np.sqrt((d ** 2).mean())
+    return x, ne0 * (1 + eps * d)
+
+
+def test_grad_stencil_matches_calc_dndr(lib, golden):
+    from turbulence_tracing_amd.particle_tracker import ElectronCube, c
+    x = golden["cube_x"]
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(golden["cube_ne"])
+    cube.calc_dndr(lwl=1064e-9)
+    for k in ("dndx", "dndy", "dndz"):
+        got = getattr(cube, k).cpu().numpy()
+        want = golden["cube_" + k]
+        assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
+    # float32 canonical-layout input with the device-side RMS normalisation
+    n = 24
+    xx, ne = small_cube(n)
+    ref = OT.Cube(xx, xx.copy(), xx.copy())
+    ref.external_ne(ne.copy())
+    ref.calc_dndr(lwl=1064e-9)
+    gx, gy, gz, eta = ref.node_fields()
+    cube2 = ElectronCube(xx, xx.copy(), xx.copy())
+    cube2.external_ne(torch.as_tensor(ne, dtype=torch.float32, device=DEV))
+    cube2.calc_dndr(lwl=1064e-9)
+    f4 = cube2.field4.double().cpu().numpy()  # [z][y][x][4]
+    dz = xx[1] - xx[0]
+    k0 = 2 * np.pi / 1064e-9
+    for ch, want in enumerate([gx * dz, gy * dz, gz * dz, eta * k0 * dz]):
+        got = f4[..., ch].transpose(2, 1, 0)
+        assert np.abs(got - want).max() <= 1e-3 * np.abs(want).max(), ch
+
+
+def test_grad_stencil_clip_and_linear_exact(lib):
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    n = 17
+    x = np.linspace(-5e-3, 5e-3, n)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.test_slab(s=8, n_e0=1e25)
+    cube.calc_dndr()
+    ref = OT.Cube(x, x.copy(), x.copy())
+    XX = np.meshgrid(x, x, x, indexing="ij")[0]
+    ref.external_ne(1e25 * (1 + 8 * XX / x.max()))
+    ref.calc_dndr()
+    assert np.abs(cube.dndx.cpu().numpy() - ref.dndx).max() <= 1e-6 * np.abs(ref.dndx).max()
+    # clipping above ne_max (PT:231): liner beyond critical density
+    cube.test_liner(n_e0=2e27, LR=1e-3)
+    cube.calc_dndr(ne_max=1)
+    ref2 = OT.Cube(x, x.copy(), x.copy())
+    ref2.external_ne(cube.ne.cpu().numpy())
+    ref2.calc_dndr(ne_max=1)
+    assert np.abs(cube.dndz.cpu().numpy() - ref2.dndz).max() <= 1e-6 * np.abs(ref2.dndz).max()
+
+
+# ----------------------------------------------------------------- tracer
+def trace_both(n=33, n_rays=4096, seed=2, eps=0.1, divergence=50e-6, beam=2e-3, h=0.5, sort=True):
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    x, ne = small_cube(n, seed=seed, eps=eps)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne)
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(n_rays, beam, divergence, seed=seed + 100)
+    rf = cube.solve(h=h, sort=sort, verbose=False).cpu().numpy()
+    ph = cube.phase.cpu().numpy()
+    sf = cube.sf.cpu().numpy()
+    ref = OT.Cube(x, x.copy(), x.copy())
+    ref.external_ne(ne)
+    ref.calc_dndr(lwl=1064e-9)
+    ref.extent = ref.extent_z
+    g = OT.NodeGrid(ref)
+    rf_o, ph_o, sf_o, st = OT.zmarch_rk4(g, s0.cpu().numpy(), h=h)
+    return x, rf, ph, sf, rf_o, ph_o, sf_o, cube
+
+
+def check_tolerance(x, rf, ph, rf_o, ph_o, s0_theta_rms=None):
+    dx = x[1] - x[0]
+    pos_err = np.abs(rf[[0, 2]] - rf_o[[0, 2]]).max() / dx
+    defl = np.sqrt(np.mean(rf_o[[1, 3]] ** 2))
+    ang_err = np.abs(rf[[1, 3]] - rf_o[[1, 3]]).max()
+    ph_err = np.abs(ph - ph_o).max()
+    assert pos_err <= 1e-3, pos_err
+    assert ang_err <= 1e-3 * defl, (ang_err, defl)
+    assert ph_err <= 1e-5 * np.abs(ph_o).max(), ph_err
+    return pos_err, ang_err / defl, ph_err
+
+
+def test_trace_matches_oracle_turbulent(lib):
+    x, rf, ph, sf, rf_o, ph_o, sf_o, _ = trace_both()
+    check_tolerance(x, rf, ph, rf_o, ph_o)
+    np.testing.assert_allclose(sf, sf_o, rtol=1e-6, atol=1e-9)
+
+
+def test_trace_unsorted_equals_sorted(lib):
+    a = trace_both(n=17, n_rays=2000, sort=True)
+    b = trace_both(n=17, n_rays=2000, sort=False)
+    np.testing.assert_array_equal(a[1], b[1])
+    np.testing.assert_array_equal(a[2], b[2])
+
+
+def test_trace_golden_cube_vs_reference_solve(lib, golden):
+    """The reference's own as-shipped trace (golden) sits within its RK45
+    tolerance of the GPU trace of the same rays."""
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    x = golden["cube_x"]
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(golden["cube_ne"])
+    cube.calc_dndr(lwl=1064e-9)
+    cube.s0 = torch.as_tensor(golden["beam_s0"], device=DEV)
+    cube.extent = cube.extent_z
+    rf = cube.solve(verbose=False).cpu().numpy()
+    ref = golden["solve_rf"]
+    dx = x[1] - x[0]
+    assert np.abs(rf[[0, 2]] - ref[[0, 2]]).max() < 1e-2 * dx
+    assert np.abs(rf[[1, 3]] - ref[[1, 3]]).max() < 1e-2 * np.abs(ref[[1, 3]]).max()
+    # and sf / ray_at_exit consistency: ray_at_exit(sf) reproduces rf
+    np.testing.assert_allclose(cube.ray_at_exit().cpu().numpy(), rf, rtol=1e-9, atol=1e-12)
+
+
+def test_trace_null_and_slab_closed_forms(lib):
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    n = 21
+    x = np.linspace(-5e-3, 5e-3, n)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.test_null()
+    cube.calc_dndr()
+    s0 = cube.init_beam(1000, 2e-3, 5e-3, seed=1)
+    rf = cube.solve(verbose=False).cpu().numpy()
+    s = s0.cpu().numpy()
+    tx, ty = s[3] / s[5], s[4] / s[5]
+    np.testing.assert_allclose(rf[0], s[0] + tx * 10e-3, atol=1e-12)
+    np.testing.assert_allclose(rf[1], np.arctan(tx), atol=1e-12)
+    assert np.all(cube.phase.cpu().numpy() == 0)
+    cube.test_slab(s=8, n_e0=1e25)
+    cube.calc_dndr()
+    cube.init_beam(1000, 1e-3, 0.0, seed=2)
+    rf = cube.solve(verbose=False).cpu().numpy()
+    nc = OT.critical_density(1053e-9)
+    slope = -0.5 * (1e25 * 8 / x.max() / nc) * 10e-3
+    np.testing.assert_allclose(rf[1], np.arctan(slope), rtol=1e-5)
+
+
+def test_trace_nan_and_backward_rays_flagged(lib):
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    n = 9
+    x = np.linspace(-5e-3, 5e-3, n)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.test_null()
+    cube.calc_dndr()
+    s0 = cube.init_beam(8, 1e-3, 1e-3, seed=3)
+    s0[0, 1] = float("nan")
+    s0[5, 2] = -s0[5, 2]
+    cube.s0 = s0
+    rf = cube.solve(verbose=False).cpu().numpy()
+    st = cube.status.cpu().numpy()
+    assert st[1] == 2 and st[2] == 1 and (st[[0, 3, 4, 5, 6, 7]] == 0).all()
+    assert np.isnan(rf[:, 1]).all() and np.isnan(rf[:, 2]).all()
+
+
+def test_dndr_and_dsdt_match_reference_interpolator(lib, golden):
+    from turbulence_tracing_amd import particle_tracker as pt
+    x = golden["cube_x"]
+    cube = pt.ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(golden["cube_ne"])
+    cube.calc_dndr(lwl=1064e-9)
+    got = cube.dndr(golden["dndr_pts"]).cpu().numpy()
+    want = golden["dndr_vals"]
+    assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
+    assert np.all(got[:, 2] == 0)
+    s = np.concatenate([golden["dndr_pts"], np.ones((3, 64))]).flatten()
+    d = pt.dsdt(0.0, s, cube).cpu().numpy().reshape(6, 64)
+    np.testing.assert_array_equal(d[:3], np.ones((3, 64)))
+
+
+# ------------------------------------------------------------------- beam
+def test_beam_sampler_matches_philox_twin(lib):
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    x = np.linspace(-5e-3, 5e-3, 9)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    s0 = cube.init_beam(5000, 2e-3, 50e-6, seed=1234).cpu().numpy()
+    ut, ua, ub, uphi, nchi = OP.beam_uniforms(np.arange(5000), 1234)
+    u = ua + ub
+    u[u > 1] = 2 - u[u > 1]
+    want = OT.beam_from_draws(2 * np.pi * ut, u, np.pi * uphi, 50e-6 * nchi, 2e-3, 5e-3, "z")
+    np.testing.assert_allclose(s0, want, rtol=1e-12, atol=1e-18)
+    r = np.hypot(s0[0], s0[1])
+    assert r.max() <= 2e-3 and abs(np.mean(r ** 2) / 2e-3 ** 2 - 0.5) < 0.02   # uniform in the disc
+    th = s0[3] / s0[5]
+    assert abs(np.var(th) / (50e-6 ** 2 / 2) - 1) < 0.1                        # Var(theta_x) = sigma^2 / 2
+
+
+def test_ray_sort_is_permutation_by_column(lib):
+    from turbulence_tracing_amd import _lib
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    n = 65
+    x = np.linspace(-5e-3, 5e-3, n)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    s0 = cube.init_beam(100000, 3e-3, 1e-3, seed=9)
+    order = cube._sort_order(s0).cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(100000))
+    s = s0.cpu().numpy()[:, order]
+    ix = np.floor((s[0] + 5e-3) / (x[1] - x[0])).astype(int)
+    iy = np.floor((s[1] + 5e-3) / (x[1] - x[0])).astype(int)
+
+    def morton(a, b):
+        k = np.zeros_like(a)
+        for bit in range(8):
+            k |= ((a >> bit) & 1) << (2 * bit) | ((b >> bit) & 1) << (2 * bit + 1)
+        return k
+    keys = morton(ix, iy)
+    assert np.all(np.diff(keys) >= 0)
+
+
+# -------------------------------------------------------------- detectors
+@pytest.mark.parametrize("name", ["sh", "df", "lf", "afr"])
+def test_detectors_match_reference_golden(lib, golden, name):
+    from turbulence_tracing_amd import ray_transfer_matrix as rtm
+    r0 = golden["rtm_r0"]
+    cls, kw = {"sh": (rtm.Shadowgraphy, {}), "df": (rtm.Schlieren_DF, {"R": 1}), "lf": (rtm.Schlieren_LF, {"R": 1}),
+               "afr": (rtm.AFR, {"Rs": np.array([0.5, 1.0, 2.0, 3.0])})}[name]
+    d = cls(r0, focal_plane=1.0, L=400, R=25, Lx=6, Ly=6)
+    d.solve(**kw)
+    d.histogram(bin_scale=1, pix_x=64, pix_y=48)
+    np.testing.assert_allclose(d.rf.cpu().numpy(), golden[f"rtm_{name}_rf"], rtol=1e-12, atol=1e-12,
+                               equal_nan=True)
+    np.testing.assert_array_equal(d.H.cpu().numpy(), golden[f"rtm_{name}_H"])
+    np.testing.assert_array_equal(d.xedges, golden[f"rtm_{name}_xedges"])
+
+
+def test_histogram_edges_exact(lib):
+    """Rays exactly on bin edges, on the last edge, outside and NaN."""
+    from turbulence_tracing_amd import ray_transfer_matrix as rtm
+    nb = 10
+    edges = np.linspace(-3, 3, nb + 1)
+    xs = np.concatenate([edges, [-3.0000001, 3.0000001, np.nan, 0.123, -2.9999999]])
+    xs_m = xs / 1e3
+    r0 = np.zeros((4, xs.size))
+    r0[0] = xs_m
+    r0[2] = 0.0
+    d = rtm.Rays(r0, Lx=6, Ly=6)
+    d.rf = d.r0
+    d.histogram(bin_scale=1, pix_x=nb, pix_y=4)
+    H_want, _, _ = OR.histogram(rtm.m_to_mm(r0).cpu().numpy(), 6, 6, 1, nb, 4)
+    np.testing.assert_array_equal(d.H.cpu().numpy(), H_want)
+
+
+def test_fused_detectors_match_oracle_on_traced_rays(lib):
+    from turbulence_tracing_amd import ray_transfer_matrix as rtm
+    x, rf, ph, sf, rf_o, ph_o, sf_o, _ = trace_both(n=17, n_rays=20000, divergence=2e-3)
+    tilt = 2 * np.pi * 20 / 6
+    Hs, Hk, Hi = rtm.detect_fused(torch.as_tensor(rf, device=DEV), torch.as_tensor(ph, device=DEV), 6, 6, 64, 64,
+                                  knife=(0.0, "x", 1), tilt_k=tilt)
+    # identical rays -> identical integer histograms
+    Ws = OR.histogram(OR.shadowgraphy(rf), 6, 6, 1, 64, 64)[0]
+    Wk = OR.histogram(OR.schlieren_knife(rf, 0.0, "x", 1), 6, 6, 1, 64, 64)[0]
+    np.testing.assert_array_equal(Hs.cpu().numpy(), Ws)
+    np.testing.assert_array_equal(Hk.cpu().numpy(), Wk)
+    sh = OR.shadowgraphy(rf)
+    Wi = OR.histogram(sh, 6, 6, 1, 64, 64, weights=OR.interferogram_weights(sh, ph, tilt))[0]
+    np.testing.assert_allclose(Hi.cpu().numpy(), Wi, rtol=1e-5, atol=1e-3)
+    # the oracle's own rays (same beam) -> counts agree within Poisson noise
+    Wo = OR.histogram(OR.shadowgraphy(rf_o), 6, 6, 1, 64, 64)[0]
+    H = Hs.cpu().numpy().astype(float)
+    m = (H + Wo) > 0
+    chi2 = np.sum((H[m] - Wo[m]) ** 2 / (H[m] + Wo[m]))
+    assert chi2 <= m.sum() * 0.05
+
+
+def test_interferogram_uniform_slab_fringes(lib):
+    """Uniform slab: phase is the same for every ray, so the fringes are
+    straight with period 2 pi / tilt_k."""
+    from turbulence_tracing_amd import ray_transfer_matrix as rtm
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    n = 9
+    x = np.linspace(-5e-3, 5e-3, n)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(np.full((n, n, n), 3e25))
+    cube.calc_dndr(lwl=1064e-9)
+    cube.init_beam(200000, 3e-3, 0.0, seed=5)
+    rf = cube.solve(verbose=False)
+    ph = cube.phase
+    r = 3e25 / OT.critical_density(1064e-9)
+    want = 2 * np.pi / 1064e-9 * (np.sqrt(1 - r) - 1) * 10e-3
+    np.testing.assert_allclose(ph.cpu().numpy(), want, rtol=1e-6)
+    tilt = 2 * np.pi / 1.0   # 1 mm period
+    Hs, Hk, Hi = rtm.detect_fused(rf, ph, 6, 6, 60, 1, tilt_k=tilt)
+    w = Hi.cpu().numpy()[0] / np.maximum(Hs.cpu().numpy()[0], 1)
+    xc = (np.arange(60) + 0.5) * 0.1 - 3
+    pred = 1 + np.cos(want - tilt * xc)
+    ok = Hs.cpu().numpy()[0] > 50
+    assert np.abs(w[ok] - pred[ok]).max() < 0.2
+
+
+# --------------------------------------------------------------- pipeline
+def test_pipeline_small_end_to_end(lib):
+    from turbulence_tracing_amd.pipeline import Config, Pipeline
+    cfg = Config(n=64, n_rays=20000, pix=128)
+    p = Pipeline(cfg)
+    p.run(seed=5)
+    torch.cuda.synchronize()
+    # field is the Philox twin, normalised in the stencil
+    want = OF.powerlaw_field((64, 64, 64), cfg.beta, seed=5)
+    got = p.field.double().cpu().numpy().transpose(2, 1, 0)
+    assert rel_rms(got, want) <= 1e-5
+    # trace of the sorted rays against the FP64 integrator on the same field
+    f4 = p.field4.double().cpu().numpy()
+    ref = OT.Cube.__new__(OT.Cube)
+    g = OT.NodeGrid.__new__(OT.NodeGrid)
+    dz = p.grid.dz
+    g.shape = (64, 64, 64)
+    g.origin = (p.grid.x0, p.grid.y0, p.grid.z0)
+    g.spacing = (p.grid.dx, p.grid.dy, p.grid.dz)
+    k0 = 2 * np.pi / cfg.lwl
+    g.fields = tuple(f4[..., ch].transpose(2, 1, 0) / (dz if ch < 3 else k0 * dz) for ch in range(4))
+    g.k0 = k0
+    g.extent = cfg.L / 2
+    s0 = p.s0.cpu().numpy()
+    rf_o, ph_o, _, _ = OT.zmarch_rk4(g, s0)
+    x = np.linspace(-cfg.L / 2, cfg.L / 2, 64)
+    check_tolerance(x, p.rf.cpu().numpy(), p.phase.cpu().numpy(), rf_o, ph_o)
+    assert int(p.H_shadow.sum()) > 0.9 * cfg.n_rays

@@ -1,0 +1,59 @@
+"""The C-ABI library loads and exports every symbol include/tt.h declares
+(no compute calls: this runs without a GPU)."""
+import os
+import re
+
+import pytest
+
+from turbulence_tracing_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    src = open(os.path.join(ROOT, "include", "tt.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(tt_\w+)\s*\(", src)))
+
+
+def test_header_matches_binding():
+    assert header_symbols() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libtt.so not built: run python -m turbulence_tracing_amd.build")
+    lib = _lib.load()
+    for name in header_symbols():
+        assert hasattr(lib.dll, name), name
+    assert lib.version() == 1
+
+
+def test_argument_errors_raise_without_gpu():
+    """Argument validation happens before any device work."""
+    lib = _lib.load()
+    g = _lib.Grid(1, 4, 4, 0, 0, 0, 1, 1, 1)
+    with pytest.raises(_lib.TTError, match="nodes"):
+        lib.tt_grad_stencil(_lib.C.c_void_p(16), 0, 1, 1, 1, _lib.C.byref(g), 0, 1, 1, 1, None, 0,
+                            _lib.C.c_void_p(16), None)
+    opts = _lib.TraceOpts(0.3, 1.0)
+    g = _lib.Grid(4, 4, 4, 0, 0, 0, 1, 1, 1)
+    with pytest.raises(_lib.TTError, match="1/m"):
+        lib.tt_trace_rk4(_lib.C.c_void_p(16), _lib.C.byref(g), _lib.C.byref(opts), _lib.C.c_void_p(16), 4, 4,
+                         None, _lib.C.c_void_p(16), 4, None, None, 4, None, None)
+    t = _lib.Train()
+    t.n_ops = 1
+    t.ops[0] = _lib.Optic(_lib.OP_KNIFE, 0, 0.0, 0.0)
+    with pytest.raises(_lib.TTError, match="direction"):
+        lib.tt_optics_apply(_lib.C.byref(t), _lib.C.c_void_p(16), 4, 4, _lib.C.c_void_p(16), 4, None)
+    with pytest.raises(_lib.TTError, match="sizes"):
+        lib.tt_fft_plan_create(1, 4, 4, _lib.C.byref(_lib.P()))
+
+
+def test_optics_op_tables_match_header():
+    src = open(os.path.join(ROOT, "include", "tt.h")).read()
+    for name, val in [("DISTANCE", 1), ("LENS", 2), ("APERTURE", 3), ("STOP", 4), ("KNIFE", 5), ("RECT", 6),
+                      ("ANNULUS", 7)]:
+        assert re.search(rf"TT_OP_{name}\s*=\s*{val}\b", src)
+        assert getattr(_lib, "OP_" + name) == val
+    assert re.search(rf"#define TT_MAX_OPS {_lib.TT_MAX_OPS}\b", src)

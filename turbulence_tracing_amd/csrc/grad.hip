@@ -1,0 +1,116 @@
+// Gradient stencil: the work of ElectronCube.calc_dndr
+// (particle_tracking/particle_tracker.py:220-241) as one HBM-bound pass.
+//
+// For each node: ne/nc = alpha + beta f, clipped at ne_max (PT:230-231);
+// -0.5 d(ne/nc)/dx_i by np.gradient's rule (central interior, one-sided
+// first-order edges; PT:235-237); the result is written as one float4 per
+// node {Gx, Gy, Gz, Phi} in the x-fastest/z-slowest layout the tracer reads:
+//   G   = -0.5 dz grad(ne/nc)          (velocity change per z-cell)
+//   Phi = k0 dz (sqrt(1 - ne/nc) - 1)  (phase per z-cell)
+// Algorithmic traffic: sizeof(T) read + 16 B written per node (neighbour
+// re-reads hit L1/L2).
+#include "tt_internal.h"
+
+namespace {
+
+template <typename T>
+struct GradArgs {
+    const T *f;
+    int64_t sx, sy, sz;
+    int nx, ny, nz;
+    T beta, clipv;  // s = min(beta f, clipv); ne/nc = alpha + s
+    double alpha;
+    T cx, cy, cz;   // -0.5 dz / (2 d_i): scale of a central difference
+    float phase_scale;  // k0 dz
+    const double *sumsq; // optional: beta /= sqrt(*sumsq / count) (unit-RMS normalisation)
+    double count;
+    float4 *out;
+};
+
+template <typename T>
+__device__ __forceinline__ T load_s(const GradArgs<T> &a, T beta, int ix, int iy, int iz) {
+    const T v = a.f[(int64_t)ix * a.sx + (int64_t)iy * a.sy + (int64_t)iz * a.sz] * beta;
+    return v < a.clipv ? v : a.clipv;
+}
+
+// one-dimensional np.gradient at index i of n given the stencil values
+template <typename T>
+__device__ __forceinline__ T grad1(T sm, T s0, T sp, int i, int n, T c) {
+    if (i == 0) return (sp - s0) * (c + c);
+    if (i == n - 1) return (s0 - sm) * (c + c);
+    return (sp - sm) * c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void grad_kernel(GradArgs<T> a) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    const int iz = blockIdx.z;
+    if (ix >= a.nx || iy >= a.ny) return;
+    T beta = a.beta;
+    if (a.sumsq) beta = (T)((double)beta / sqrt(*a.sumsq / a.count));
+    const T s0 = load_s(a, beta, ix, iy, iz);
+    const T xm = ix > 0 ? load_s(a, beta, ix - 1, iy, iz) : s0;
+    const T xp = ix < a.nx - 1 ? load_s(a, beta, ix + 1, iy, iz) : s0;
+    const T ym = iy > 0 ? load_s(a, beta, ix, iy - 1, iz) : s0;
+    const T yp = iy < a.ny - 1 ? load_s(a, beta, ix, iy + 1, iz) : s0;
+    const T zm = iz > 0 ? load_s(a, beta, ix, iy, iz - 1) : s0;
+    const T zp = iz < a.nz - 1 ? load_s(a, beta, ix, iy, iz + 1) : s0;
+    float4 o;
+    o.x = (float)grad1(xm, s0, xp, ix, a.nx, a.cx);
+    o.y = (float)grad1(ym, s0, yp, iy, a.ny, a.cy);
+    o.z = (float)grad1(zm, s0, zp, iz, a.nz, a.cz);
+    // n - 1 = -r / (1 + sqrt(1 - r)), evaluated without cancellation
+    const double r = a.alpha + (double)s0;
+    const double q = 1.0 - r;
+    const double eta = q > 0.0 ? -r / (1.0 + sqrt(q)) : -1.0;
+    o.w = (float)(eta * (double)a.phase_scale);
+    a.out[((int64_t)iz * a.ny + iy) * a.nx + ix] = o;
+}
+
+template <typename T>
+int launch(const void *field, int64_t sx, int64_t sy, int64_t sz, const tt_grid *g, double alpha, double beta,
+           double ne_max, double k0, const double *sumsq, int64_t count, float *out4, hipStream_t st) {
+    GradArgs<T> a;
+    a.f = static_cast<const T *>(field);
+    a.sx = sx;
+    a.sy = sy;
+    a.sz = sz;
+    a.nx = g->nx;
+    a.ny = g->ny;
+    a.nz = g->nz;
+    a.beta = (T)beta;
+    a.clipv = (T)(ne_max - alpha);
+    a.alpha = alpha;
+    a.cx = (T)(-0.5 * g->dz / (2.0 * g->dx));
+    a.cy = (T)(-0.5 * g->dz / (2.0 * g->dy));
+    a.cz = (T)(-0.5 * g->dz / (2.0 * g->dz));
+    a.phase_scale = (float)(k0 * g->dz);
+    a.sumsq = sumsq;
+    a.count = (double)count;
+    a.out = reinterpret_cast<float4 *>(out4);
+    dim3 block(64, 4, 1);
+    dim3 grid((unsigned)tt::ceil_div(g->nx, 64), (unsigned)tt::ceil_div(g->ny, 4), (unsigned)g->nz);
+    hipLaunchKernelGGL(grad_kernel<T>, grid, block, 0, st, a);
+    TT_LAUNCH_CHECK("tt_grad_stencil launch");
+    return TT_OK;
+}
+
+}  // namespace
+
+extern "C" int tt_grad_stencil(const void *field, int32_t dtype, int64_t sx, int64_t sy, int64_t sz,
+                               const tt_grid *grid, double alpha, double beta, double ne_max, double k0,
+                               const double *sumsq, int64_t count, float *out4, void *stream) {
+    TT_REQUIRE(field && grid && out4, "tt_grad_stencil: null pointer");
+    TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2,
+               "tt_grad_stencil: every axis needs >= 2 nodes (np.gradient), got %d x %d x %d", grid->nx,
+               grid->ny, grid->nz);
+    TT_REQUIRE(grid->nz <= 65535, "tt_grad_stencil: nz > 65535");
+    TT_REQUIRE(grid->dx > 0 && grid->dy > 0 && grid->dz > 0, "tt_grad_stencil: spacing must be > 0");
+    TT_REQUIRE(!sumsq || count > 0, "tt_grad_stencil: count must be > 0 with sumsq");
+    hipStream_t st = tt::as_stream(stream);
+    if (dtype == 0) return launch<float>(field, sx, sy, sz, grid, alpha, beta, ne_max, k0, sumsq, count, out4, st);
+    if (dtype == 1) return launch<double>(field, sx, sy, sz, grid, alpha, beta, ne_max, k0, sumsq, count, out4, st);
+    tt::set_error("tt_grad_stencil: dtype must be 0 (float32) or 1 (float64), got %d", dtype);
+    return TT_ERR_ARG;
+}

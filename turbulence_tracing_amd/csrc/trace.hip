@@ -1,0 +1,280 @@
+// Ray integrator: the work of ElectronCube.solve + ray_at_exit and of every
+// dsdt/dndr evaluation inside solve_ivp
+// (particle_tracking/particle_tracker.py:243-256, 312-380, 398-419).
+//
+// The reference integrates d x/dt = v, d v/dt = -(c^2/2) grad(ne/nc) in
+// time with an adaptive RK45 shared by all rays.  Here each ray is one lane
+// and the same ODE is integrated with z (in cells) as the independent
+// variable by classic RK4 with a fixed step of h = 1/m cells:
+//   d px/dZ = (dz/dx) U/W   d py/dZ = (dz/dy) V/W
+//   d U/dZ  = Gx / W        d V/dZ  = Gy / W        d W/dZ = Gz / W
+//   d tau/dZ = dz / W       d phase/dZ = Phi |v| / W
+// (U, V, W) = v/c and (Gx, Gy, Gz, Phi) is the trilinear interpolant of the
+// float4 node field written by tt_grad_stencil (zero strictly outside the
+// grid, like RegularGridInterpolator with fill_value=0).  Steps land on the
+// z-planes, so each step stays inside one z-cell and the last lands exactly
+// on the exit plane z = z.max() that ray_at_exit back-projects to.
+//
+// Precision: positions are carried as an fp64 base plus an fp32 displacement
+// from the entry point, W as W - 1, and the phase is summed per step into an
+// fp64 accumulator.
+//
+// Algorithmic traffic (roofline accounting): 4 stages x 8 corners x 16 B =
+// 512 B per ray-step.
+#include "tt_internal.h"
+
+namespace {
+
+struct TraceArgs {
+    const float4 *field;
+    int nx, ny, nz;
+    float fx1, fy1, fz1;  // n - 1 as float (upper bounds, inclusive)
+    double x0, y0, z0, dx, dy, dz;
+    float sx, sy;         // dz/dx, dz/dy
+    int m;                // steps per cell
+    float h;              // 1/m
+    double cT;
+    const double *s0;
+    int64_t ld_s0, n;
+    const int32_t *order;
+    double *rf;
+    int64_t ld_rf;
+    double *phase;
+    double *sf;
+    int64_t ld_sf;
+    int32_t *status;
+};
+
+struct State {
+    float dpx, dpy, U, V, Wd, tr;  // Wd = W - 1, tr = int (1/W - 1) dZ
+};
+
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+    return make_float4(fmaf(t, b.x - a.x, a.x), fmaf(t, b.y - a.y, a.y), fmaf(t, b.z - a.z, a.z),
+                       fmaf(t, b.w - a.w, a.w));
+}
+
+// Trilinear sample in cell coordinates; zero strictly outside [0, n-1]^3.
+__device__ __forceinline__ float4 sample(const TraceArgs &a, float px, float py, float pz) {
+    if (!(px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1 && pz >= 0.f && pz <= a.fz1))
+        return make_float4(0.f, 0.f, 0.f, 0.f);
+    const int ix = min((int)px, a.nx - 2);
+    const int iy = min((int)py, a.ny - 2);
+    const int iz = min((int)pz, a.nz - 2);
+    const float tx = px - (float)ix, ty = py - (float)iy, tz = pz - (float)iz;
+    const int64_t plane = (int64_t)a.nx * a.ny;
+    const float4 *p = a.field + ((int64_t)iz * a.ny + iy) * a.nx + ix;
+    const float4 c000 = p[0], c100 = p[1], c010 = p[a.nx], c110 = p[a.nx + 1];
+    const float4 c001 = p[plane], c101 = p[plane + 1], c011 = p[plane + a.nx], c111 = p[plane + a.nx + 1];
+    const float4 c00 = lerp4(c000, c100, tx), c10 = lerp4(c010, c110, tx);
+    const float4 c01 = lerp4(c001, c101, tx), c11 = lerp4(c011, c111, tx);
+    return lerp4(lerp4(c00, c10, ty), lerp4(c01, c11, ty), tz);
+}
+
+__device__ __forceinline__ void rhs(const TraceArgs &a, float pxb, float pyb, float Z, const State &s, State &k,
+                                    float &kph) {
+    const float4 g = sample(a, pxb + s.dpx, pyb + s.dpy, Z);
+    const float W = 1.0f + s.Wd;
+    const float iW = 1.0f / W;
+    k.dpx = a.sx * s.U * iW;
+    k.dpy = a.sy * s.V * iW;
+    k.U = g.x * iW;
+    k.V = g.y * iW;
+    k.Wd = g.z * iW;
+    k.tr = -s.Wd * iW;
+    kph = g.w * sqrtf(fmaf(s.U, s.U, fmaf(s.V, s.V, W * W))) * iW;
+}
+
+__device__ __forceinline__ State axpy(const State &y, float c, const State &k) {
+    return State{fmaf(c, k.dpx, y.dpx), fmaf(c, k.dpy, y.dpy), fmaf(c, k.U, y.U),
+                 fmaf(c, k.V, y.V),     fmaf(c, k.Wd, y.Wd),   fmaf(c, k.tr, y.tr)};
+}
+
+// one classic RK4 step of size hh from Z; returns the phase increment
+__device__ __forceinline__ float rk4_step(const TraceArgs &a, float pxb, float pyb, float Z, float hh, State &y) {
+    State k1, k2, k3, k4;
+    float p1, p2, p3, p4;
+    const float h2 = 0.5f * hh;
+    rhs(a, pxb, pyb, Z, y, k1, p1);
+    rhs(a, pxb, pyb, Z + h2, axpy(y, h2, k1), k2, p2);
+    rhs(a, pxb, pyb, Z + h2, axpy(y, h2, k2), k3, p3);
+    rhs(a, pxb, pyb, Z + hh, axpy(y, hh, k3), k4, p4);
+    const float h6 = hh * (1.0f / 6.0f);
+    y.dpx += h6 * (k1.dpx + 2.f * (k2.dpx + k3.dpx) + k4.dpx);
+    y.dpy += h6 * (k1.dpy + 2.f * (k2.dpy + k3.dpy) + k4.dpy);
+    y.U += h6 * (k1.U + 2.f * (k2.U + k3.U) + k4.U);
+    y.V += h6 * (k1.V + 2.f * (k2.V + k3.V) + k4.V);
+    y.Wd += h6 * (k1.Wd + 2.f * (k2.Wd + k3.Wd) + k4.Wd);
+    y.tr += h6 * (k1.tr + 2.f * (k2.tr + k3.tr) + k4.tr);
+    return h6 * (p1 + 2.f * (p2 + p3) + p4);
+}
+
+__global__ __launch_bounds__(256) void trace_rk4_kernel(TraceArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int64_t r = a.order ? (int64_t)a.order[t] : t;
+    const double *s = a.s0 + r;
+    const double x = s[0], y = s[a.ld_s0], z = s[2 * a.ld_s0];
+    const double vx = s[3 * a.ld_s0], vy = s[4 * a.ld_s0], vz = s[5 * a.ld_s0];
+    int st = 0;
+    if (isnan(x) || isnan(y) || isnan(z) || isnan(vx) || isnan(vy) || isnan(vz))
+        st = 2;
+    else if (!(vz > 0.0))
+        st = 1;
+
+    const double U0 = vx / tt::kC, V0 = vy / tt::kC, W0 = vz / tt::kC;
+    double px = (x - a.x0) / a.dx, py = (y - a.y0) / a.dy, Zs = (z - a.z0) / a.dz;
+    double tau = 0.0;
+    const double Zend = (double)(a.nz - 1);
+    State ys{0.f, 0.f, (float)U0, (float)V0, (float)(W0 - 1.0), 0.f};
+    double ph = 0.0;
+
+    if (st == 0) {
+        // field is zero before the first plane: carry the ray straight there
+        if (Zs < 0.0) {
+            const double d = -Zs;
+            px += (double)a.sx * U0 / W0 * d;
+            py += (double)a.sy * V0 / W0 * d;
+            tau += a.dz * d / W0;
+            Zs = 0.0;
+        }
+        if (Zs >= Zend) {
+            // starts at or past the exit plane: straight back-projection
+            const double d = Zend - Zs;
+            px += (double)a.sx * U0 / W0 * d;
+            py += (double)a.sy * V0 / W0 * d;
+            tau += a.dz * d / W0;
+        } else {
+            const float pxb = (float)px, pyb = (float)py;
+            const int K = (a.nz - 1) * a.m;
+            const int kstart = (int)ceil(Zs * a.m - 1e-9);
+            const double h0 = kstart * (double)a.h - Zs;
+            if (h0 > 1e-12) ph += (double)rk4_step(a, pxb, pyb, (float)Zs, (float)h0, ys);
+            for (int k = kstart; k < K; ++k) {
+                ph += (double)rk4_step(a, pxb, pyb, (float)k * a.h, a.h, ys);
+                if (!(ys.Wd > -1.0f)) {  // turned back: z is no longer monotone
+                    st = 1;
+                    break;
+                }
+            }
+            px += (double)ys.dpx;
+            py += (double)ys.dpy;
+            tau += a.dz * ((Zend - Zs) + (double)ys.tr);
+        }
+    }
+
+    const double nanv = __longlong_as_double(0x7ff8000000000000ll);
+    const double U = ys.U, V = ys.V, W = 1.0 + (double)ys.Wd;
+    const double xe = a.x0 + px * a.dx, ye = a.y0 + py * a.dy, ze = a.z0 + Zend * a.dz;
+    const bool ok = st == 0;
+    double *rf = a.rf + r;
+    rf[0] = ok ? xe : nanv;
+    rf[a.ld_rf] = ok ? atan(U / W) : nanv;
+    rf[2 * a.ld_rf] = ok ? ye : nanv;
+    rf[3 * a.ld_rf] = ok ? atan(V / W) : nanv;
+    if (a.phase) a.phase[r] = ok ? ph : nanv;
+    if (a.sf) {
+        const double rem = a.cT - tau;
+        double *sf = a.sf + r;
+        sf[0] = ok ? xe + U * rem : nanv;
+        sf[a.ld_sf] = ok ? ye + V * rem : nanv;
+        sf[2 * a.ld_sf] = ok ? ze + W * rem : nanv;
+        sf[3 * a.ld_sf] = ok ? tt::kC * U : nanv;
+        sf[4 * a.ld_sf] = ok ? tt::kC * V : nanv;
+        sf[5 * a.ld_sf] = ok ? tt::kC * W : nanv;
+    }
+    if (a.status) a.status[r] = st;
+}
+
+// trilinear samples of the node field at arbitrary points (metres)
+__global__ __launch_bounds__(256) void sample_kernel(TraceArgs a, const double *pts, int64_t ld, int64_t n,
+                                                     double *out, int64_t ldo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pts[i], y = pts[ld + i], z = pts[2 * ld + i];
+    const double nanv = __longlong_as_double(0x7ff8000000000000ll);
+    if (isnan(x) || isnan(y) || isnan(z)) {
+        for (int c = 0; c < 4; ++c) out[c * ldo + i] = nanv;
+        return;
+    }
+    // bounds in double (RegularGridInterpolator compares coordinates)
+    const double px = (x - a.x0) / a.dx, py = (y - a.y0) / a.dy, pz = (z - a.z0) / a.dz;
+    const bool in = px >= 0.0 && px <= (double)a.fx1 && py >= 0.0 && py <= (double)a.fy1 && pz >= 0.0 &&
+                    pz <= (double)a.fz1;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in) g = sample(a, fminf((float)px, a.fx1), fminf((float)py, a.fy1), fminf((float)pz, a.fz1));
+    out[i] = g.x;
+    out[ldo + i] = g.y;
+    out[2 * ldo + i] = g.z;
+    out[3 * ldo + i] = g.w;
+}
+
+int fill_grid_args(const tt_grid *grid, TraceArgs &a) {
+    TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "grid needs >= 2 nodes per axis");
+    TT_REQUIRE(grid->dx > 0 && grid->dy > 0 && grid->dz > 0, "grid spacing must be > 0");
+    a.nx = grid->nx;
+    a.ny = grid->ny;
+    a.nz = grid->nz;
+    a.fx1 = (float)(grid->nx - 1);
+    a.fy1 = (float)(grid->ny - 1);
+    a.fz1 = (float)(grid->nz - 1);
+    a.x0 = grid->x0;
+    a.y0 = grid->y0;
+    a.z0 = grid->z0;
+    a.dx = grid->dx;
+    a.dy = grid->dy;
+    a.dz = grid->dz;
+    a.sx = (float)(grid->dz / grid->dx);
+    a.sy = (float)(grid->dz / grid->dy);
+    return TT_OK;
+}
+
+}  // namespace
+
+extern "C" int tt_sample_field(const float *field4, const tt_grid *grid, const double *pts, int64_t ld, int64_t n,
+                               double *out, int64_t ld_out, void *stream) {
+    TT_REQUIRE(field4 && grid && pts && out, "tt_sample_field: null pointer");
+    TT_REQUIRE(n >= 0 && ld >= n && ld_out >= n, "tt_sample_field: bad sizes");
+    if (n == 0) return TT_OK;
+    TraceArgs a{};
+    if (int rc = fill_grid_args(grid, a)) return rc;
+    a.field = reinterpret_cast<const float4 *>(field4);
+    hipLaunchKernelGGL(sample_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream), a,
+                       pts, ld, n, out, ld_out);
+    TT_LAUNCH_CHECK("tt_sample_field launch");
+    return TT_OK;
+}
+
+extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
+                            int64_t ld_s0, int64_t n, const int32_t *order, double *rf, int64_t ld_rf,
+                            double *phase, double *sf, int64_t ld_sf, int32_t *status, void *stream) {
+    TT_REQUIRE(field4 && grid && opts && s0 && rf, "tt_trace_rk4: null pointer");
+    TT_REQUIRE(n >= 0, "tt_trace_rk4: n < 0");
+    if (n == 0) return TT_OK;
+    TT_REQUIRE(ld_s0 >= n && ld_rf >= n && (!sf || ld_sf >= n), "tt_trace_rk4: leading dimension < n");
+    const double mm = 1.0 / opts->h;
+    const int m = (int)llround(mm);
+    TT_REQUIRE(m >= 1 && m <= 64 && fabs(mm - m) < 1e-9, "tt_trace_rk4: h must be 1/m cells with 1 <= m <= 64");
+    TT_REQUIRE((int64_t)(grid->nz - 1) * m < (1 << 23), "tt_trace_rk4: too many steps for exact float Z");
+    TraceArgs a{};
+    if (int rc = fill_grid_args(grid, a)) return rc;
+    a.field = reinterpret_cast<const float4 *>(field4);
+    a.m = m;
+    a.h = 1.0f / (float)m;
+    a.cT = opts->cT;
+    a.s0 = s0;
+    a.ld_s0 = ld_s0;
+    a.n = n;
+    a.order = order;
+    a.rf = rf;
+    a.ld_rf = ld_rf;
+    a.phase = phase;
+    a.sf = sf;
+    a.ld_sf = ld_sf;
+    a.status = status;
+    const int block = 256;
+    hipLaunchKernelGGL(trace_rk4_kernel, dim3((unsigned)tt::ceil_div(n, block)), dim3(block), 0,
+                       tt::as_stream(stream), a);
+    TT_LAUNCH_CHECK("tt_trace_rk4 launch");
+    return TT_OK;
+}

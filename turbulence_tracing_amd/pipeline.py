@@ -1,0 +1,153 @@
+"""One realisation of the hot path on the GPU, with every buffer resident.
+
+field (tt_field_generate: Philox noise x sqrt P(k), inverse 3-D FFT, sum of
+squares) -> node field (tt_grad_stencil with the unit-RMS normalisation and
+ne = ne0 (1 + eps delta) mapping applied on the device) -> beam
+(tt_beam_sample) -> ray order (tt_ray_sort, rays gathered into entry-column
+order) -> trace (tt_trace_rk4) -> shadowgraph, knife-edge schlieren and
+interferogram histograms in one pass (tt_detect).
+
+This composes, on one device and without host round trips, what a user of
+the reference composes by hand: gaussian3D_FFT (TG:488-538), normalisation
+and the density map (caller code), ElectronCube.external_ne / calc_dndr /
+init_beam / solve (PT:212-331) and the ray_transfer_matrix detectors
+(RTM:156-250).
+"""
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .particle_tracker import NC_COEFF, c
+from .ray_transfer_matrix import make_train, hist_desc, op_knife, schlieren_ops, shadowgraphy_ops
+from .turboGen import FieldPlan
+
+
+@dataclass
+class Config:
+    n: int = 512                 # grid nodes per axis
+    L: float = 10e-3             # cube side, m (grid spans [-L/2, L/2])
+    n_rays: int = 10_000_000
+    ne0: float = 1e24            # mean electron density, m^-3
+    eps: float = 0.1             # RMS fluctuation of ne / ne0
+    lwl: float = 1064e-9         # probe wavelength, m
+    beta: float = -11.0 / 3.0    # spectral index of P(k)
+    aniso: float = 1.0
+    beam_size: float = 2e-3      # beam radius, m
+    divergence: float = 50e-6    # rad
+    pix: int = 2048              # detector bins per axis
+    det_size: float = 6.0        # detector side, mm
+    L_optics: float = 400.0      # mm
+    R_optics: float = 25.0       # mm
+    fringes: float = 20.0        # interferogram fringes across the detector
+    h: float = 0.5               # RK4 step, cells
+
+    @property
+    def nc(self):
+        return NC_COEFF * (2 * np.pi * c / self.lwl) ** 2
+
+    @property
+    def steps_per_ray(self):
+        return int(round((self.n - 1) / self.h))
+
+
+class Pipeline:
+    def __init__(self, cfg: Config, device="cuda"):
+        self.cfg = cfg
+        self.device = device
+        self.lib = _lib.load()
+        n = cfg.n
+        dx = cfg.L / (n - 1)
+        self.grid = _lib.Grid(n, n, n, -cfg.L / 2, -cfg.L / 2, -cfg.L / 2, dx, dx, dx)
+        self.plan = FieldPlan((n, n, n), device)
+        f32, f64 = torch.float32, torch.float64
+        self.field = torch.empty((n, n, n), dtype=f32, device=device)
+        self.sumsq = torch.zeros(1, dtype=f64, device=device)
+        self.field4 = torch.empty((n, n, n, 4), dtype=f32, device=device)
+        N = cfg.n_rays
+        self.s0_raw = torch.empty((6, N), dtype=f64, device=device)
+        self.s0 = torch.empty((6, N), dtype=f64, device=device)
+        self.order = torch.empty(N, dtype=torch.int32, device=device)
+        nbytes = _lib.C.c_size_t()
+        self.lib.tt_ray_sort_scratch_bytes(_lib.C.byref(self.grid), 0, _lib.C.byref(nbytes))
+        self.sort_scratch = torch.empty(nbytes.value, dtype=torch.uint8, device=device)
+        self.rf = torch.empty((4, N), dtype=f64, device=device)
+        self.phase = torch.empty(N, dtype=f64, device=device)
+        P = cfg.pix
+        self.H_shadow = torch.zeros((P, P), dtype=torch.int32, device=device)
+        self.H_knife = torch.zeros((P, P), dtype=torch.int32, device=device)
+        self.H_interf = torch.zeros((P, P), dtype=f32, device=device)
+        self.opts = _lib.TraceOpts(cfg.h, float(np.sqrt(8.0) * cfg.L / 2))
+        self._dets = self._detectors()
+
+    def _detectors(self):
+        cfg = self.cfg
+        sh = shadowgraphy_ops(cfg.L_optics, cfg.R_optics, 0.0)
+        ke = schlieren_ops(cfg.L_optics, cfg.R_optics, 0.0, [op_knife(0.0, "x", 1)])
+        tilt = 2 * np.pi * cfg.fringes / cfg.det_size
+        arr = (_lib.Detector * 3)()
+        for k, (ops, mode, tk, H) in enumerate([(sh, 0, 0.0, self.H_shadow), (ke, 0, 0.0, self.H_knife),
+                                                (sh, 1, tilt, self.H_interf)]):
+            arr[k].train = make_train(ops, to_mm=True)
+            arr[k].hist = hist_desc(cfg.det_size, cfg.det_size, cfg.pix, cfg.pix)
+            arr[k].mode = mode
+            arr[k].tilt_k = tk
+            if mode == 0:
+                arr[k].counts = H.data_ptr()
+            else:
+                arr[k].wsum = H.data_ptr()
+        return arr
+
+    # individual stages (all asynchronous on the current stream) -------------
+    def generate_field(self, seed):
+        cfg = self.cfg
+        self.sumsq.zero_()
+        self.plan.generate(cfg.beta, cfg.aniso, 1.0 / cfg.n, 0.5, 1.0, seed, out=self.field, sumsq=self.sumsq)
+
+    def gradient(self):
+        cfg = self.cfg
+        n = cfg.n
+        alpha = cfg.ne0 / cfg.nc
+        beta = cfg.ne0 * cfg.eps / cfg.nc
+        self.lib.tt_grad_stencil(_lib.ptr(self.field), 0, 1, n, n * n, _lib.C.byref(self.grid), alpha, beta, 1.0,
+                                 2 * np.pi / cfg.lwl, _lib.ptr(self.sumsq), n ** 3, _lib.ptr(self.field4),
+                                 _lib.stream_ptr())
+
+    def beam(self, seed):
+        cfg = self.cfg
+        N = cfg.n_rays
+        self.lib.tt_beam_sample(N, cfg.beam_size, cfg.divergence, cfg.L / 2, 2, int(seed), 0, _lib.ptr(self.s0_raw),
+                                N, _lib.stream_ptr())
+
+    def sort(self):
+        N = self.cfg.n_rays
+        self.lib.tt_ray_sort(_lib.ptr(self.s0_raw), N, N, _lib.C.byref(self.grid), 0, _lib.ptr(self.sort_scratch),
+                             _lib.ptr(self.order), _lib.stream_ptr())
+        torch.index_select(self.s0_raw, 1, self.order.long(), out=self.s0)
+
+    def trace(self):
+        N = self.cfg.n_rays
+        self.lib.tt_trace_rk4(_lib.ptr(self.field4), _lib.C.byref(self.grid), _lib.C.byref(self.opts),
+                              _lib.ptr(self.s0), N, N, None, _lib.ptr(self.rf), N, _lib.ptr(self.phase), None, N,
+                              None, _lib.stream_ptr())
+
+    def detect(self):
+        N = self.cfg.n_rays
+        self.H_shadow.zero_()
+        self.H_knife.zero_()
+        self.H_interf.zero_()
+        self.lib.tt_detect(self._dets, 3, _lib.ptr(self.rf), N, _lib.ptr(self.phase), N, _lib.stream_ptr())
+
+    def run(self, seed=0):
+        """One realisation: field seed ``seed``, beam seed ``seed + 1``."""
+        self.generate_field(seed)
+        self.gradient()
+        self.beam(seed + 1)
+        self.sort()
+        self.trace()
+        self.detect()
+
+    @property
+    def ray_steps(self):
+        return self.cfg.n_rays * self.cfg.steps_per_ray
