@@ -36,8 +36,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--rays", type=float, default=1e7)
-    ap.add_argument("--cpu-rays-per-proc", type=int, default=1500)
+    ap.add_argument("--cpu-bundle", type=int, default=500, help="rays per process per CPU round")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU rounds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="tracer kernel: 0 cell-cached, 1 per-stage gathers")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (reference kernel for HBM counters)")
     return ap.parse_args()
 
 
@@ -63,7 +67,7 @@ def main():
 
     from turbulence_tracing_amd.pipeline import Config, Pipeline
 
-    cfg = Config(n=args.n, n_rays=int(args.rays))
+    cfg = Config(n=args.n, n_rays=int(args.rays), variant=args.variant)
     p = Pipeline(cfg)
     log(f"pipeline ready: {cfg.n}^3, {cfg.n_rays} rays")
     stream = torch.cuda.current_stream()
@@ -99,6 +103,15 @@ def main():
     ray_steps = p.ray_steps
     value = ray_steps / (ms_per_step / 1e3)
     achieved = BYTES_PER_RAY_STEP * ray_steps / (trace_ms / 1e3) / 1e9
+
+    if args.calib_copy:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy: 1 GiB read + 1 GiB written")
 
     # stage breakdown (separate, un-timed-region pass with events per stage)
     stages = {}
@@ -149,15 +162,15 @@ def main():
     }
 
     if base is not None:
-        log(f"cpu baseline: {base.P} processes x {args.cpu_rays_per_proc} rays")
+        log(f"cpu baseline: {base.P} processes, bundles of {args.cpu_bundle} rays, ~{args.cpu_budget:.0f} s")
         try:
-            out["cpu_baseline"] = cpu_baseline(base, p, cfg, args.cpu_rays_per_proc)
+            out["cpu_baseline"] = cpu_baseline(base, p, cfg, args.cpu_bundle, args.cpu_budget)
         finally:
             base.close()
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(base, p, cfg, per_proc):
+def cpu_baseline(base, p, cfg, bundle, budget):
     import numpy as np
     import torch
     c = 299792458.0
@@ -169,10 +182,10 @@ def cpu_baseline(base, p, cfg, per_proc):
     del f4
     base.load_gradients(grads[0], grads[1], grads[2], x, x, x)
     del grads
-    nray = base.P * per_proc
-    s0 = p.s0_raw[:, :nray].cpu().numpy()
+    s0 = p.s0_raw[:, :base.P * bundle * 64].cpu().numpy()
     torch.cuda.synchronize()
-    rf, wall, nfev = base.run(s0, base.P)
+    log("cpu baseline: gradients in shared memory")
+    nray, wall, nfev = base.run_timed(s0, bundle, budget, log)
     rays_per_s = nray / wall
     return {
         "value": rays_per_s * cfg.steps_per_ray,
@@ -180,9 +193,9 @@ def cpu_baseline(base, p, cfg, per_proc):
         "rays_per_s": rays_per_s,
         "cores": base.P,
         "kind": "port",
-        "sample": f"{nray} rays of the same beam through the same {cfg.n}^3 field: {base.P} processes x "
-                  f"{per_proc} rays, one as-shipped solve_ivp RK45 (rtol 1e-3) per process; ray-steps "
-                  f"counted as rays x {cfg.steps_per_ray} (the GPU's steps per ray); wall {wall:.2f} s, "
+        "sample": f"{nray} rays of the same beam through the same {cfg.n}^3 field, in rounds of {base.P} "
+                  f"processes x {bundle} rays, one as-shipped solve_ivp RK45 (rtol 1e-3) per bundle; "
+                  f"ray-steps counted as rays x {cfg.steps_per_ray} (the GPU's steps per ray); wall {wall:.2f} s, "
                   f"nfev {nfev}",
     }
 

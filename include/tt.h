@@ -106,10 +106,13 @@ int tt_grad_stencil(const void *field, int32_t dtype, int64_t sx, int64_t sy, in
  * (e.g. from tt_ray_sort); outputs are written at each ray's own index.
  * sf (nullable) receives the state at c t = cT, continued straight past the
  * exit plane (the field is zero outside the grid, PT:239-241).
- * status (nullable, int32): 0 ok, 1 ray not moving forward in z, 2 NaN. */
+ * status (nullable, int32): 0 ok, 1 ray not moving forward in z, 2 NaN.
+ * variant 0: each lane caches its current cell as a trilinear polynomial in
+ * registers (the fast path); variant 1: eight float4 gathers per RK4 stage. */
 typedef struct {
     double h;
     double cT;
+    int32_t variant;
 } tt_trace_opts;
 int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *opts,
                  const double *s0, int64_t ld_s0, int64_t n, const int32_t *order,

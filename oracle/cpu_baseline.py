@@ -89,6 +89,24 @@ class CPUBaseline:
         rf = np.concatenate([o[0] for o in out], axis=1)
         return rf, wall, sum(o[2] for o in out)
 
+    def run_timed(self, s0, bundle, budget_s, log=None):
+        """Rounds of P bundles of ``bundle`` rays (one per process) until the
+        wall time reaches ``budget_s`` or the rays run out.  Returns
+        (rays done, wall seconds, total nfev)."""
+        done, nfev, wall = 0, 0, 0.0
+        N = s0.shape[1]
+        r = 0
+        while done + self.P * bundle <= N and wall < budget_s:
+            chunk = s0[:, done:done + self.P * bundle]
+            _, w, nf = self.run(chunk, self.P)
+            done += chunk.shape[1]
+            wall += w
+            nfev += nf
+            r += 1
+            if log:
+                log(f"cpu baseline round {r}: {chunk.shape[1]} rays in {w:.2f} s")
+        return done, wall, nfev
+
     def close(self):
         self.pool.close()
         self.pool.join()

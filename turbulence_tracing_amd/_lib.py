@@ -34,7 +34,7 @@ class Spectrum(C.Structure):
 
 
 class TraceOpts(C.Structure):
-    _fields_ = [("h", C.c_double), ("cT", C.c_double)]
+    _fields_ = [("h", C.c_double), ("cT", C.c_double), ("variant", C.c_int32)]
 
 
 class Optic(C.Structure):

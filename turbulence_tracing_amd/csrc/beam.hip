@@ -25,19 +25,19 @@ __global__ __launch_bounds__(256) void beam_kernel(int64_t n, double beam_size, 
     const double chi = divergence * nrm;
     const double sc = sin(chi), cc = cos(chi), sp = sin(phi), cp = cos(phi);
     const double pa = beam_size * u * cos(t), pb = beam_size * u * sin(t);
-    double v[6];
-    if (axis == 0) {  // PT:280-289
-        v[3] = tt::kC * cc; v[4] = tt::kC * sc * cp; v[5] = tt::kC * sc * sp;
-        v[0] = extent; v[1] = pa; v[2] = pb;
-    } else if (axis == 1) {  // PT:290-299
-        v[4] = tt::kC * cc; v[3] = tt::kC * sc * cp; v[5] = tt::kC * sc * sp;
-        v[0] = pa; v[1] = -extent; v[2] = pb;
-    } else {  // PT:300-309
-        v[3] = tt::kC * sc * cp; v[4] = tt::kC * sc * sp; v[5] = tt::kC * cc;
-        v[0] = pa; v[1] = pb; v[2] = -extent;
-    }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) s0[c * ld + i] = v[c];
+    // PT:280-309: axis 0 (x) starts at +extent, axes 1 and 2 at -extent; the
+    // along-axis velocity is c cos(chi), the two transverse ones
+    // c sin(chi) cos(phi) and c sin(chi) sin(phi).  Written as per-row selects
+    // (a branchy local array of six doubles was mis-lowered for gfx950).
+    const double vl = tt::kC * cc, va = tt::kC * sc * cp, vb = tt::kC * sc * sp;
+    const bool ax0 = axis == 0, ax1 = axis == 1;
+    double *o = s0 + i;
+    o[0] = ax0 ? extent : pa;
+    o[ld] = ax0 ? pa : (ax1 ? -extent : pb);
+    o[2 * ld] = ax0 ? pb : (ax1 ? pb : -extent);
+    o[3 * ld] = ax0 ? vl : va;
+    o[4 * ld] = ax0 ? va : (ax1 ? vl : vb);
+    o[5 * ld] = ax0 ? vb : (ax1 ? vb : vl);
 }
 
 // ---------------------------------------------------------------- ray sort

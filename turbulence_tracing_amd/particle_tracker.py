@@ -217,7 +217,7 @@ class ElectronCube:
                              _lib.ptr(order), _lib.stream_ptr())
         return order
 
-    def solve(self, method="RK45", h=0.5, sort=True, verbose=True):
+    def solve(self, method="RK45", h=0.5, sort=True, verbose=True, variant=0):
         """Trace all rays through the cube and project them to the exit plane
         (PT:312-331).  Returns rf (4 x N: x, theta, y, phi)."""
         if self.probing_direction != "z":
@@ -231,7 +231,7 @@ class ElectronCube:
         self.sf = torch.empty((6, n), dtype=torch.float64, device=self.device)
         self.phase = torch.empty(n, dtype=torch.float64, device=self.device)
         self.status = torch.empty(n, dtype=torch.int32, device=self.device)
-        opts = _lib.TraceOpts(float(h), float(np.sqrt(8.0) * extent))
+        opts = _lib.TraceOpts(float(h), float(np.sqrt(8.0) * extent), int(variant))
         start = time.time()
         order = self._sort_order(s0) if (sort and n > 0) else None
         self.lib.tt_trace_rk4(_lib.ptr(self.field4), _lib.C.byref(self.grid), _lib.C.byref(opts), _lib.ptr(s0), n,

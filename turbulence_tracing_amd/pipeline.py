@@ -42,6 +42,7 @@ class Config:
     R_optics: float = 25.0       # mm
     fringes: float = 20.0        # interferogram fringes across the detector
     h: float = 0.5               # RK4 step, cells
+    variant: int = 0             # tracer kernel: 0 cell-cached, 1 per-stage gathers
 
     @property
     def nc(self):
@@ -78,7 +79,7 @@ class Pipeline:
         self.H_shadow = torch.zeros((P, P), dtype=torch.int32, device=device)
         self.H_knife = torch.zeros((P, P), dtype=torch.int32, device=device)
         self.H_interf = torch.zeros((P, P), dtype=f32, device=device)
-        self.opts = _lib.TraceOpts(cfg.h, float(np.sqrt(8.0) * cfg.L / 2))
+        self.opts = _lib.TraceOpts(cfg.h, float(np.sqrt(8.0) * cfg.L / 2), cfg.variant)
         self._dets = self._detectors()
 
     def _detectors(self):
