@@ -110,34 +110,26 @@ __device__ __forceinline__ float rk4_step(const TraceArgs &a, float pxb, float p
 }
 
 // ---------------------------------------------------------------------------
-// Cell-cached evaluation.  Each lane keeps the cell its ray is in as the
+// Cell-cached evaluation.  A lane loads the cell its ray is in once, as the
 // trilinear polynomial f = a + b tx + c ty + d tz + e tx ty + f tx tz +
-// g ty tz + h tx ty tz (8 float4 coefficients in registers) and reloads it
-// only when a stage position leaves the cell: every 1/h steps in z (all lanes
-// together) and rarely in x/y.  A stage then costs 7 FMAs per channel and no
-// memory access, instead of 8 float4 gathers.  Outside the grid the cached
-// coefficients are zero (cell key -2).
+// g ty tz + h tx ty tz (8 float4 coefficients in registers), and runs every
+// RK4 step of that z-cell from it: a stage costs 7 FMAs per channel and no
+// memory access instead of 8 float4 gathers.  Each stage checks that its point
+// lies in the cell; when one does not (the ray crosses an x/y face, or lies
+// outside the grid in x/y), the step is redone with per-stage gathers and the
+// cell is located again.  The cell is (re)loaded at the top of the outer loop
+// only, so the coefficients are never merged across a branch.
 // ---------------------------------------------------------------------------
 struct Cell {
     float4 a, b, c, d, e, f, g, h;
-    int ix, iy, iz;
 };
 
 __device__ __forceinline__ float4 f4sub(float4 p, float4 q) { return make_float4(p.x - q.x, p.y - q.y, p.z - q.z, p.w - q.w); }
-__device__ __forceinline__ float4 f4add(float4 p, float4 q) { return make_float4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w); }
 __device__ __forceinline__ float4 f4fma(float4 p, float t, float4 q) {
     return make_float4(fmaf(p.x, t, q.x), fmaf(p.y, t, q.y), fmaf(p.z, t, q.z), fmaf(p.w, t, q.w));
 }
 
 __device__ __forceinline__ void load_cell(const TraceArgs &A, int ix, int iy, int iz, Cell &C) {
-    C.ix = ix;
-    C.iy = iy;
-    C.iz = iz;
-    if (ix < 0) {
-        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-        C.a = C.b = C.c = C.d = C.e = C.f = C.g = C.h = z;
-        return;
-    }
     const int64_t plane = (int64_t)A.nx * A.ny;
     const float4 *p = A.field + ((int64_t)iz * A.ny + iy) * A.nx + ix;
     const float4 c000 = p[0], c100 = p[1], c010 = p[A.nx], c110 = p[A.nx + 1];
@@ -159,50 +151,107 @@ __device__ __forceinline__ float4 eval_cell(const Cell &C, float tx, float ty, f
     return f4fma(f4fma(E, ty, B), tx, f4fma(Cc, ty, A));
 }
 
-// field value at (px, py) in cells and z-cell iz with fraction tz
-__device__ __forceinline__ float4 sample_cached(const TraceArgs &a, Cell &C, float px, float py, int iz, float tz) {
-    const bool in = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
-    const int ix = in ? min((int)px, a.nx - 2) : -2;
-    const int iy = in ? min((int)py, a.ny - 2) : -2;
-    if (ix != C.ix || iy != C.iy || iz != C.iz) load_cell(a, ix, iy, iz, C);
-    return eval_cell(C, px - (float)ix, py - (float)iy, tz);
-}
-
-__device__ __forceinline__ void rhs_c(const TraceArgs &a, Cell &C, float pxb, float pyb, int iz, float tz,
-                                      const State &s, State &k, float &kph) {
-    const float4 g = sample_cached(a, C, pxb + s.dpx, pyb + s.dpy, iz, tz);
+// one RK4 stage from the cached cell; ok &= the stage point lies in the cell
+template <bool TAU>
+__device__ __forceinline__ void rhs_c(const TraceArgs &a, const Cell &C, float ox, float oy, float tz, const State &s,
+                                      State &k, float &kph, bool &ok) {
+    const float tx = ox + s.dpx, ty = oy + s.dpy;  // ox = pxb - ix
+    ok = ok && tx >= 0.f && tx <= 1.f && ty >= 0.f && ty <= 1.f;
+    const float4 g = eval_cell(C, tx, ty, tz);
     const float W = 1.0f + s.Wd;
-    const float iW = 1.0f / W;
+    const float iW = __builtin_amdgcn_rcpf(W);
     k.dpx = a.sx * s.U * iW;
     k.dpy = a.sy * s.V * iW;
     k.U = g.x * iW;
     k.V = g.y * iW;
     k.Wd = g.z * iW;
-    k.tr = -s.Wd * iW;
-    kph = g.w * sqrtf(fmaf(s.U, s.U, fmaf(s.V, s.V, W * W))) * iW;
+    k.tr = TAU ? -s.Wd * iW : 0.f;
+    kph = g.w * __builtin_amdgcn_sqrtf(fmaf(s.U, s.U, fmaf(s.V, s.V, W * W))) * iW;
 }
 
-// RK4 step of size hh from Z (inside z-cell iz, tz0 = Z - iz)
-__device__ __forceinline__ float rk4_step_c(const TraceArgs &a, Cell &C, float pxb, float pyb, int iz, float tz0,
-                                            float hh, State &y) {
-    State k1, k2, k3, k4;
-    float p1, p2, p3, p4;
+template <bool TAU>
+__device__ __forceinline__ float rk4_step_c(const TraceArgs &a, const Cell &C, float ox, float oy, float tz0, float hh,
+                                            State &y, bool &ok) {
+    // running sum acc = k1 + 2 k2 + 2 k3 + k4 keeps one stage's slopes live
+    State k, acc;
+    float p, pacc;
     const float h2 = 0.5f * hh;
-    rhs_c(a, C, pxb, pyb, iz, tz0, y, k1, p1);
-    rhs_c(a, C, pxb, pyb, iz, tz0 + h2, axpy(y, h2, k1), k2, p2);
-    rhs_c(a, C, pxb, pyb, iz, tz0 + h2, axpy(y, h2, k2), k3, p3);
-    rhs_c(a, C, pxb, pyb, iz, tz0 + hh, axpy(y, hh, k3), k4, p4);
+    rhs_c<TAU>(a, C, ox, oy, tz0, y, k, p, ok);
+    acc = k;
+    pacc = p;
+    rhs_c<TAU>(a, C, ox, oy, tz0 + h2, axpy(y, h2, k), k, p, ok);
+    acc = axpy(acc, 2.f, k);
+    pacc = fmaf(2.f, p, pacc);
+    rhs_c<TAU>(a, C, ox, oy, tz0 + h2, axpy(y, h2, k), k, p, ok);
+    acc = axpy(acc, 2.f, k);
+    pacc = fmaf(2.f, p, pacc);
+    rhs_c<TAU>(a, C, ox, oy, tz0 + hh, axpy(y, hh, k), k, p, ok);
+    acc = axpy(acc, 1.f, k);
+    pacc += p;
     const float h6 = hh * (1.0f / 6.0f);
-    y.dpx += h6 * (k1.dpx + 2.f * (k2.dpx + k3.dpx) + k4.dpx);
-    y.dpy += h6 * (k1.dpy + 2.f * (k2.dpy + k3.dpy) + k4.dpy);
-    y.U += h6 * (k1.U + 2.f * (k2.U + k3.U) + k4.U);
-    y.V += h6 * (k1.V + 2.f * (k2.V + k3.V) + k4.V);
-    y.Wd += h6 * (k1.Wd + 2.f * (k2.Wd + k3.Wd) + k4.Wd);
-    y.tr += h6 * (k1.tr + 2.f * (k2.tr + k3.tr) + k4.tr);
-    return h6 * (p1 + 2.f * (p2 + p3) + p4);
+    y.dpx = fmaf(h6, acc.dpx, y.dpx);
+    y.dpy = fmaf(h6, acc.dpy, y.dpy);
+    y.U = fmaf(h6, acc.U, y.U);
+    y.V = fmaf(h6, acc.V, y.V);
+    y.Wd = fmaf(h6, acc.Wd, y.Wd);
+    if (TAU) y.tr = fmaf(h6, acc.tr, y.tr);
+    return h6 * pacc;
 }
 
-template <bool CACHED>
+// Steps k .. K-1 (z = k h) from the lattice, cell-cached.  Every lane loads
+// its cell (indices clamped to the grid, so a lane outside the grid or with a
+// NaN position still reads valid memory) and runs the fast step; when a stage
+// of any lane leaves its cell the wave takes one uniform branch in which the
+// failing lanes redo the step with gathers, then every lane locates its cell
+// again.  The cached coefficients are dead inside that branch, so the gather
+// path does not add to the register footprint of the fast path.  Returns
+// false if the ray turned back (W <= 0).
+template <bool TAU>
+__device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
+                                             double &ph) {
+    bool good = true;
+    while (k < K) {
+        const int iz = k / a.m;
+        const int kend = min(K, (iz + 1) * a.m);
+        const float px = pxb + ys.dpx, py = pyb + ys.dpy;
+        const bool inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
+        const int ix = inside ? min((int)px, a.nx - 2) : 0;
+        const int iy = inside ? min((int)py, a.ny - 2) : 0;
+        Cell C;
+        load_cell(a, ix, iy, iz, C);
+        const float ox = pxb - (float)ix, oy = pyb - (float)iy;
+        bool redo = false, ok = true;
+        State yn;
+        float dph = 0.f;
+        while (k < kend) {
+            const float tz0 = (float)(k - iz * a.m) * a.h;
+            ok = inside;
+            yn = ys;
+            dph = rk4_step_c<TAU>(a, C, ox, oy, tz0, a.h, yn, ok);
+            if (__ballot(!ok) != 0) {  // wave-uniform: redo step k below
+                redo = true;
+                break;
+            }
+            ys = yn;
+            ph += (double)dph;
+            ++k;
+            good = good && (ys.Wd > -1.0f);
+        }
+        if (redo) {  // the cell is dead here; failing lanes gather
+            if (!ok) {
+                yn = ys;
+                dph = rk4_step(a, pxb, pyb, (float)k * a.h, a.h, yn);
+            }
+            ys = yn;
+            ph += (double)dph;
+            ++k;
+            good = good && (ys.Wd > -1.0f);
+        }
+    }
+    return good;
+}
+
+template <bool CACHED, bool TAU>
 __global__ __launch_bounds__(256) void trace_rk4_kernel(TraceArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
@@ -244,21 +293,9 @@ __global__ __launch_bounds__(256) void trace_rk4_kernel(TraceArgs a) {
             const int kstart = (int)ceil(Zs * a.m - 1e-9);
             const double h0 = kstart * (double)a.h - Zs;
             if (CACHED) {
-                Cell C;
-                C.ix = C.iy = C.iz = -3;  // nothing cached yet
-                if (h0 > 1e-12) {
-                    const int iz = min((int)Zs, a.nz - 2);
-                    ph += (double)rk4_step_c(a, C, pxb, pyb, iz, (float)(Zs - iz), (float)h0, ys);
-                }
-                for (int k = kstart; k < K; ++k) {
-                    const int iz = k / a.m;
-                    const float tz0 = (float)(k - iz * a.m) * a.h;
-                    ph += (double)rk4_step_c(a, C, pxb, pyb, iz, tz0, a.h, ys);
-                    if (!(ys.Wd > -1.0f)) {  // turned back: z is no longer monotone
-                        st = 1;
-                        break;
-                    }
-                }
+                // partial first step (off-lattice start) with gathers, then the lattice
+                if (h0 > 1e-12) ph += (double)rk4_step(a, pxb, pyb, (float)Zs, (float)h0, ys);
+                if (!march_cached<TAU>(a, pxb, pyb, kstart, K, ys, ph)) st = 1;
             } else {
                 if (h0 > 1e-12) ph += (double)rk4_step(a, pxb, pyb, (float)Zs, (float)h0, ys);
                 for (int k = kstart; k < K; ++k) {
@@ -386,12 +423,15 @@ extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_t
     a.status = status;
     const int block = 256;
     TT_REQUIRE(opts->variant == 0 || opts->variant == 1, "tt_trace_rk4: variant must be 0 or 1");
-    if (opts->variant == 0)
-        hipLaunchKernelGGL(trace_rk4_kernel<true>, dim3((unsigned)tt::ceil_div(n, block)), dim3(block), 0,
-                           tt::as_stream(stream), a);
+    const dim3 nblk((unsigned)tt::ceil_div(n, block));
+    hipStream_t st = tt::as_stream(stream);
+    // the time integral is needed only for sf
+    if (opts->variant == 0 && sf)
+        hipLaunchKernelGGL((trace_rk4_kernel<true, true>), nblk, dim3(block), 0, st, a);
+    else if (opts->variant == 0)
+        hipLaunchKernelGGL((trace_rk4_kernel<true, false>), nblk, dim3(block), 0, st, a);
     else
-        hipLaunchKernelGGL(trace_rk4_kernel<false>, dim3((unsigned)tt::ceil_div(n, block)), dim3(block), 0,
-                           tt::as_stream(stream), a);
+        hipLaunchKernelGGL((trace_rk4_kernel<false, true>), nblk, dim3(block), 0, st, a);
     TT_LAUNCH_CHECK("tt_trace_rk4 launch");
     return TT_OK;
 }
