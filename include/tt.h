@@ -194,6 +194,73 @@ typedef struct {
 int tt_detect(const tt_detector *dets, int32_t ndet, const double *rf, int64_t ld, const double *phase,
               int64_t n, void *stream);
 
+/* -------------------------------------------------------------- surrogate
+ * The learned correction.  None of these has a counterpart in the reference
+ * (it has no surrogate); they serve the cheap straight-line estimate, the
+ * MLP that corrects it, and its training. */
+
+/* Straight-line (second-order Born) estimate of each ray's exit state and its
+ * 48 surrogate features.  base: 5 x ldb float64 [x, theta, y, phi, phase];
+ * feat: n rows of ldf float32 (48 used): 16 z-segment sums of Gx, 16 of Gy,
+ * then S_x, S_y, M_x, M_y, T_x, T_y, R_x, R_y, S_z, phase, U0, V0, W0 - 1,
+ * entry-cell fractions x, y, and the path length in cells. */
+int tt_straight_features(const float *field4, const tt_grid *grid, const double *s0, int64_t ld, int64_t n,
+                         float *feat, int64_t ldf, double *base, int64_t ldb, void *stream);
+
+/* Apply the MLP output Y (n rows, ld ldy >= 7, fp32) to the straight-line
+ * estimate: rf = base[0:4] + Y[:, {0, 2, 1, 3}] * {s_pos, s_ang, s_pos, s_ang}
+ * and phase = base[4] + Y[:, 4] * s_ph, with scales = {s_pos, s_ang, s_ph,
+ * e_pos, e_ang}.  A ray is flagged for the full tracer when
+ * |Y[:, 5]| e_pos > thresholds[0] (metres) or |Y[:, 6]| e_ang > thresholds[1]
+ * (radians) or the estimate is not finite; flag (optional) gets 0/1 per ray
+ * and the flagged indices are appended to list (optional) with *nflag (device
+ * int32, caller-zeroed) counting them. */
+int tt_surrogate_apply(const double *base, int64_t ldb, const float *Y, int64_t ldy, int64_t n, const float *scales,
+                       const double *thresholds, double *rf, int64_t ldr, double *phase, int32_t *flag,
+                       int32_t *list, int32_t *nflag, void *stream);
+
+/* C[m][n] = act(sum_k A(m,k) B(k,n) + bias[n]) on bf16 MFMA with fp32
+ * accumulation; A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] (bf16).
+ * gelu: 1 applies GELU (erf form); Zpre (optional) keeps the pre-activation;
+ * outputs fp32 Cf and/or bf16 Cb with row stride ldc.  ksplit > 0 splits K
+ * into chunks whose raw products are ADDED atomically into Cf (no bias,
+ * activation or bf16 output). */
+typedef struct {
+    const void *A;
+    int64_t sam, sak;
+    const void *B;
+    int64_t sbk, sbn;
+    int32_t M, N, K;
+    const float *bias;
+    int32_t gelu;
+    float *Cf;
+    void *Cb;
+    int64_t ldc;
+    float *Zpre;
+    int32_t ksplit;
+} tt_gemm_args;
+int tt_gemm_bf16(const tt_gemm_args *args, void *stream);
+
+/* dZ (bf16) = dH (fp32 or bf16) * GELU'(Zpre), elementwise over n */
+int tt_gelu_backward(const float *dH_f32, const void *dH_bf16, const float *Zpre, void *dZ_bf16, int64_t n,
+                     void *stream);
+/* db[c] += sum over rows of dZ[r][c] (bf16 input, fp32 accumulate) */
+int tt_bias_grad(const void *dZ_bf16, int64_t rows, int32_t cols, int64_t ld, float *db, void *stream);
+/* Surrogate loss: weighted squared error on columns [0, n_reg) of Y against
+ * T, plus L1 error heads (column n_reg + h predicts max |T - Y| over columns
+ * [err_lo[h], err_hi[h])).  Writes the gradient / rows to dY (bf16, ld ldd,
+ * zero up to n_pad) and ADDS the two loss sums to loss_out (2 doubles). */
+int tt_surrogate_loss(const float *Y, int64_t ldy, const float *T, int64_t ldt, void *dY_bf16, int64_t ldd,
+                      int64_t rows, int32_t n_reg, const float *w_reg, int32_t n_err, const int32_t *err_lo,
+                      const int32_t *err_hi, int32_t n_pad, double *loss_out, void *stream);
+/* Adam (bias-corrected, step >= 1) on fp32 master weights; refreshes the
+ * bf16 copy w_bf16 when non-NULL. */
+int tt_adam(float *w, const float *g, float *m, float *v, void *w_bf16, int64_t n, double lr, double b1, double b2,
+            double eps, int64_t step, double weight_decay, void *stream);
+/* out (bf16, [rows][ldo], zero-padded) = (x - mu) * inv_sd over cols */
+int tt_standardize_bf16(const float *x, int64_t ldx, int64_t rows, int32_t cols, const float *mu,
+                        const float *inv_sd, void *out_bf16, int64_t ldo, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

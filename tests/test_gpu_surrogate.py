@@ -1,0 +1,171 @@
+"""Surrogate kernels against the FP64 oracle: the MFMA GEMM in every stride
+mode the MLP uses, MLP forward/backward/Adam, the straight-line estimate,
+the trained network's gain over the estimate alone, and the fallback."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import field as OF
+from oracle import surrogate as OS
+from oracle import tracer as OT
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def bf16(a):
+    return torch.as_tensor(np.asarray(a), dtype=torch.float32).to(torch.bfloat16)
+
+
+def f64(t):
+    return t.float().cpu().numpy().astype(np.float64)
+
+
+def gemm(lib, A, sam, sak, B, sbk, sbn, M, N, K, **kw):
+    from turbulence_tracing_amd import _lib
+    Cf = kw.pop("Cf")
+    a = _lib.GemmArgs(A.data_ptr(), sam, sak, B.data_ptr(), sbk, sbn, M, N, K, None, 0, Cf.data_ptr(), None,
+                      kw.pop("ldc", N), None, kw.pop("ksplit", 0))
+    lib.tt_gemm_bf16(C.byref(a), _lib.stream_ptr())
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (100, 72, 48), (1000, 256, 256), (37, 8, 256), (257, 48, 8)])
+def test_gemm_all_stride_modes(lib, M, N, K):
+    rs = np.random.RandomState(M + N + K)
+    A = rs.randn(M, K)
+    B = rs.randn(K, N)
+    Ab, Bb = bf16(A), bf16(B)
+    want = f64(Ab) @ f64(Bb)
+    scale = np.abs(want).max()
+    # A row-major (k contiguous), B given as [N][K] (k contiguous): forward layout
+    A1 = Ab.contiguous().to(DEV)
+    B1 = Bb.T.contiguous().to(DEV)
+    out = torch.zeros((M, N), dtype=torch.float32, device=DEV)
+    gemm(lib, A1, K, 1, B1, 1, K, M, N, K, Cf=out)
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=0, atol=1e-5 * scale * np.sqrt(K))
+    # A given as [K][M] (m contiguous), B row-major [K][N] (n contiguous): dW layout, split K
+    A2 = Ab.T.contiguous().to(DEV)
+    B2 = Bb.contiguous().to(DEV)
+    out.zero_()
+    gemm(lib, A2, 1, M, B2, N, 1, M, N, K, Cf=out, ksplit=32)
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=0, atol=1e-5 * scale * np.sqrt(K))
+    # element-wise fallback (odd strides): A with a padded leading dimension
+    A3 = torch.zeros((M, K + 3), dtype=torch.bfloat16)
+    A3[:, :K] = Ab
+    A3 = A3.to(DEV)
+    out.zero_()
+    gemm(lib, A3, K + 3, 1, B2, N, 1, M, N, K, Cf=out)
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=0, atol=1e-5 * scale * np.sqrt(K))
+
+
+def emulate_forward(Ws, bs, X):
+    """The GPU's numerics: bf16 weights/inputs, fp32-exact products, bf16
+    hidden activations."""
+    H = f64(bf16(X))
+    cache = []
+    for l, (W, b) in enumerate(zip(Ws, bs)):
+        Z = H @ f64(bf16(W)).T + b
+        cache.append((H, Z))
+        if l < len(Ws) - 1:
+            H = f64(bf16(OS.gelu(Z)))
+        else:
+            H = Z
+    return H, cache
+
+
+def test_mlp_forward_backward_adam_match_oracle(lib):
+    from turbulence_tracing_amd.surrogate import MLP
+    dims = (48, 256, 256, 256, 8)
+    mlp = MLP(dims, seed=3)
+    rs = np.random.RandomState(0)
+    N = 512
+    X = rs.randn(N, 48)
+    Ws = [w.cpu().numpy().astype(np.float64) for w in mlp.W]
+    bs = [b.cpu().numpy().astype(np.float64) for b in mlp.b]
+    Y = mlp.forward(bf16(X).to(DEV), keep=True).cpu().numpy()
+    Ye, cache = emulate_forward(Ws, bs, X)
+    assert np.sqrt(np.mean((Y - Ye) ** 2) / np.mean(Ye ** 2)) < 2e-3
+    G = rs.randn(N, 8) / N
+    mlp.backward(bf16(G).to(DEV))
+    torch.cuda.synchronize()
+    # oracle backward with the GPU's bf16 upstream gradients
+    params = [(f64(bf16(W)), b) for W, b in zip(Ws, bs)]
+    grads = OS.mlp_backward(params, cache, f64(bf16(G)))
+    for l in range(len(Ws)):
+        gW = mlp.dW[l].cpu().numpy()
+        rel = np.sqrt(np.mean((gW - grads[l][0]) ** 2) / np.mean(grads[l][0] ** 2))
+        assert rel < 2e-2, (l, rel)
+        gb = mlp.db[l].cpu().numpy()
+        relb = np.sqrt(np.mean((gb - grads[l][1]) ** 2) / np.mean(grads[l][1] ** 2))
+        assert relb < 2e-2, (l, relb)
+    # Adam: one step against the oracle on the GPU's own gradients
+    w_before = [w.cpu().numpy().astype(np.float64) for w in mlp.W]
+    g_now = [g.cpu().numpy().astype(np.float64) for g in mlp.dW]
+    mlp.adam(1e-3)
+    for l in range(len(Ws)):
+        w1, _, _ = OS.adam_step(w_before[l], g_now[l], 0 * w_before[l], 0 * w_before[l], 1, 1e-3)
+        np.testing.assert_allclose(mlp.W[l].cpu().numpy(), w1, rtol=1e-5, atol=1e-7)
+        np.testing.assert_array_equal(mlp.Wb[l].cpu(), mlp.W[l].cpu().to(torch.bfloat16))
+
+
+def small_cube(n=33, seed=3):
+    x = np.linspace(-5e-3, 5e-3, n)
+    d = OF.powerlaw_field((n, n, n), -11 / 3, seed=seed)
+    d /= np.sqrt((d ** 2).mean())
+    return x, 1e24 * (1 + 0.1 * d)
+
+
+def test_straight_features_match_oracle(lib):
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    from turbulence_tracing_amd.surrogate import straight
+    x, ne = small_cube()
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne)
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(2000, 2e-3, 50e-6, seed=9)
+    base, feat = straight(cube.field4, cube.grid, s0)
+    ref = OT.Cube(x, x.copy(), x.copy())
+    ref.external_ne(ne)
+    ref.calc_dndr(lwl=1064e-9)
+    ref.extent = ref.extent_z
+    bo, fo = OS.straight_estimate(OT.NodeGrid(ref), s0.cpu().numpy())
+    b = base.cpu().numpy()
+    dx = x[1] - x[0]
+    assert np.abs(b[[0, 2]] - bo[[0, 2]]).max() < 1e-5 * dx
+    assert np.abs(b[[1, 3]] - bo[[1, 3]]).max() < 1e-6 * np.abs(bo[[1, 3]]).max()
+    assert np.abs(b[4] - bo[4]).max() < 1e-5 * np.abs(bo[4]).max()
+    f = feat.cpu().numpy()
+    scale = np.abs(fo).max(axis=0) + 1e-30
+    assert (np.abs(f - fo).max(axis=0) / scale).max() < 1e-3
+
+
+def test_surrogate_gain_and_fallback(lib):
+    """Trained on rays the tracer generates, the network reduces the error of
+    the straight-line estimate on held-out fields; with a zero threshold every
+    ray falls back and the result is the full trace."""
+    from turbulence_tracing_amd.pipeline import Config, Pipeline
+    from turbulence_tracing_amd.surrogate import (Surrogate, build_dataset, errors, make_norm, normalise_targets,
+                                                  residual, straight)
+    cfg = Config(n=64, pix=64)
+    feat, resid, _ = build_dataset(12, 40000, cfg, seed=100)
+    norm = make_norm(feat, resid)
+    sur = Surrogate(norm, seed=1)
+    sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=2e-3)
+    # held-out field
+    p = Pipeline(Config(n=64, pix=64, n_rays=50000))
+    p.run(seed=999)
+    base, f2 = straight(p.field4, p.grid, p.s0)
+    Y = sur.predict(f2)
+    rf, ph, flag, lst, cnt = sur.apply(base, Y, t_pos=1.0, t_ang=1.0)
+    e_base = errors(torch.stack([base[0], base[1], base[2], base[3]]), base[4], p.rf, p.phase)
+    e_mlp = errors(rf, ph, p.rf, p.phase)
+    rms = lambda e: float(e.double().pow(2).mean().sqrt())
+    assert rms(e_mlp[1]) < 0.7 * rms(e_base[1])      # angle
+    assert rms(e_mlp[0]) < 0.7 * rms(e_base[0])      # position
+    # fallback forced from the host: every ray re-traced -> exactly the tracer
+    rf2, ph2, flag2 = sur.trace(p.field4, p.grid, p.s0, t_pos=0.0, t_ang=0.0)
+    assert int(flag2.sum()) == p.s0.shape[1]
+    np.testing.assert_array_equal(rf2.cpu().numpy(), p.rf.cpu().numpy())

@@ -55,6 +55,14 @@ class Detector(C.Structure):
                 ("counts", C.c_void_p), ("wsum", C.c_void_p)]
 
 
+class GemmArgs(C.Structure):
+    _fields_ = [("A", C.c_void_p), ("sam", C.c_int64), ("sak", C.c_int64),
+                ("B", C.c_void_p), ("sbk", C.c_int64), ("sbn", C.c_int64),
+                ("M", C.c_int32), ("N", C.c_int32), ("K", C.c_int32),
+                ("bias", C.c_void_p), ("gelu", C.c_int32), ("Cf", C.c_void_p), ("Cb", C.c_void_p),
+                ("ldc", C.c_int64), ("Zpre", C.c_void_p), ("ksplit", C.c_int32)]
+
+
 P = C.c_void_p
 I32, I64, U64, D = C.c_int32, C.c_int64, C.c_uint64, C.c_double
 SIZE_P = C.POINTER(C.c_size_t)
@@ -74,6 +82,14 @@ _SIGNATURES = {
     "tt_ray_sort": [P, I64, I64, C.POINTER(Grid), I32, P, P, P],
     "tt_optics_apply": [C.POINTER(Train), P, I64, I64, P, I64, P],
     "tt_detect": [C.POINTER(Detector), I32, P, I64, P, I64, P],
+    "tt_straight_features": [P, C.POINTER(Grid), P, I64, I64, P, I64, P, I64, P],
+    "tt_surrogate_apply": [P, I64, P, I64, I64, P, P, P, I64, P, P, P, P, P],
+    "tt_gemm_bf16": [C.POINTER(GemmArgs), P],
+    "tt_gelu_backward": [P, P, P, P, I64, P],
+    "tt_bias_grad": [P, I64, I32, I64, P, P],
+    "tt_surrogate_loss": [P, I64, P, I64, P, I64, I64, I32, P, I32, P, P, I32, P, P],
+    "tt_adam": [P, P, P, P, P, I64, D, D, D, D, I64, D, P],
+    "tt_standardize_bf16": [P, I64, I64, I32, P, P, P, I64, P],
 }
 
 # every symbol include/tt.h declares

@@ -1,0 +1,359 @@
+"""Learned correction of the straight-line ray estimate, with the full RK4
+tracer as fallback and judge.
+
+Not part of the reference (which has no surrogate).  The pieces:
+  * ``tt_straight_features`` follows each ray along its unperturbed line and
+    returns a second-order (Born) estimate of the exit state
+    [x, theta, y, phi, phase] plus 48 features;
+  * a 4-layer MLP (48 -> 256 -> 256 -> 256 -> 8, GELU, bf16 weights and
+    activations, fp32 accumulation; hand-written MFMA GEMMs in libtt)
+    predicts the residual (full trace - estimate) for x, y, theta, phi,
+    phase and two error bounds (position, angle);
+  * rays whose predicted error bound exceeds the threshold (1 um / 1 urad by
+    default), or whose estimate is not finite, are re-traced with
+    ``tt_trace_rk4``.
+Training data are rays the tracer itself generates through seeded fields.
+"""
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from .pipeline import Config, Pipeline
+
+NFEAT = 48
+NOUT = 8  # 5 residuals, 2 error bounds, 1 pad
+N_REG, N_ERR = 5, 2
+
+
+def _p(t):
+    return None if t is None else t.data_ptr()
+
+
+class MLP:
+    """Weights [out][in] fp32 (master) + bf16 copies, Adam state, and the
+    forward / backward passes as libtt GEMM calls."""
+
+    def __init__(self, dims=(NFEAT, 256, 256, 256, NOUT), device="cuda", seed=0):
+        self.dims = tuple(dims)
+        self.device = device
+        self.lib = _lib.load()
+        g = torch.Generator().manual_seed(seed)
+        self.W, self.b, self.Wb = [], [], []
+        for i in range(len(dims) - 1):
+            fan_in, fan_out = dims[i], dims[i + 1]
+            bound = 1.0 / np.sqrt(fan_in)
+            W = (torch.rand(fan_out, fan_in, generator=g, dtype=torch.float64) * 2 - 1) * bound
+            b = (torch.rand(fan_out, generator=g, dtype=torch.float64) * 2 - 1) * bound
+            self.W.append(W.float().to(device).contiguous())
+            self.b.append(b.float().to(device).contiguous())
+            self.Wb.append(self.W[-1].to(torch.bfloat16).contiguous())
+        self.mW = [torch.zeros_like(w) for w in self.W]
+        self.vW = [torch.zeros_like(w) for w in self.W]
+        self.mb = [torch.zeros_like(b) for b in self.b]
+        self.vb = [torch.zeros_like(b) for b in self.b]
+        self.dW = [torch.zeros_like(w) for w in self.W]
+        self.db = [torch.zeros_like(b) for b in self.b]
+        self.step = 0
+
+    @property
+    def nlayers(self):
+        return len(self.W)
+
+    def _gemm(self, A, sam, sak, B, sbk, sbn, M, N, K, bias=None, gelu=False, Cf=None, Cb=None, ldc=None, Zpre=None,
+              ksplit=0):
+        a = _lib.GemmArgs(_p(A), sam, sak, _p(B), sbk, sbn, M, N, K, _p(bias), 1 if gelu else 0, _p(Cf), _p(Cb),
+                          ldc if ldc is not None else N, _p(Zpre), ksplit)
+        self.lib.tt_gemm_bf16(C.byref(a), _lib.stream_ptr())
+
+    def forward(self, X, out=None, keep=False):
+        """X: bf16 [N, dims[0]] contiguous.  Returns fp32 [N, dims[-1]];
+        keep=True stores activations and pre-activations for backward."""
+        N = X.shape[0]
+        H = X
+        self.cache = [] if keep else None
+        for l in range(self.nlayers):
+            fin, fout = self.dims[l], self.dims[l + 1]
+            last = l == self.nlayers - 1
+            if last:
+                Y = out if out is not None else torch.empty((N, fout), dtype=torch.float32, device=self.device)
+                self._gemm(H, fin, 1, self.Wb[l], 1, fin, N, fout, fin, bias=self.b[l], Cf=Y, ldc=fout)
+                if keep:
+                    self.cache.append((H, None))
+                return Y
+            Hn = torch.empty((N, fout), dtype=torch.bfloat16, device=self.device)
+            Z = torch.empty((N, fout), dtype=torch.float32, device=self.device) if keep else None
+            self._gemm(H, fin, 1, self.Wb[l], 1, fin, N, fout, fin, bias=self.b[l], gelu=True, Cb=Hn, ldc=fout,
+                       Zpre=Z)
+            if keep:
+                self.cache.append((H, Z))
+            H = Hn
+
+    def backward(self, dY):
+        """dY: bf16 [N, dims[-1]] (gradient of the loss w.r.t. the output).
+        Fills self.dW / self.db."""
+        N = dY.shape[0]
+        dZ = dY
+        ks = 1024 if N >= 2048 else ((N + 31) // 32) * 32
+        for l in range(self.nlayers - 1, -1, -1):
+            fin, fout = self.dims[l], self.dims[l + 1]
+            H, _ = self.cache[l]
+            self.dW[l].zero_()
+            self.db[l].zero_()
+            # dW[out][in] = sum_n dZ[n][out] H[n][in]
+            self._gemm(dZ, 1, fout, H, fin, 1, fout, fin, N, Cf=self.dW[l], ldc=fin, ksplit=ks)
+            self.lib.tt_bias_grad(_p(dZ), N, fout, fout, _p(self.db[l]), _lib.stream_ptr())
+            if l > 0:
+                dH = torch.empty((N, fin), dtype=torch.float32, device=self.device)
+                # dH[n][in] = sum_out dZ[n][out] W[out][in]
+                self._gemm(dZ, fout, 1, self.Wb[l], fin, 1, N, fin, fout, Cf=dH, ldc=fin)
+                Zp = self.cache[l - 1][1]
+                dZn = torch.empty((N, fin), dtype=torch.bfloat16, device=self.device)
+                self.lib.tt_gelu_backward(_p(dH), None, _p(Zp), _p(dZn), N * fin, _lib.stream_ptr())
+                dZ = dZn
+
+    def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        self.step += 1
+        st = _lib.stream_ptr()
+        for l in range(self.nlayers):
+            self.lib.tt_adam(_p(self.W[l]), _p(self.dW[l]), _p(self.mW[l]), _p(self.vW[l]), _p(self.Wb[l]),
+                             self.W[l].numel(), lr, b1, b2, eps, self.step, wd, st)
+            self.lib.tt_adam(_p(self.b[l]), _p(self.db[l]), _p(self.mb[l]), _p(self.vb[l]), None, self.b[l].numel(),
+                             lr, b1, b2, eps, self.step, 0.0, st)
+
+    def state(self):
+        return {f"W{l}": self.W[l].cpu().numpy() for l in range(self.nlayers)} | \
+               {f"b{l}": self.b[l].cpu().numpy() for l in range(self.nlayers)}
+
+    def load_state(self, st):
+        for l in range(self.nlayers):
+            self.W[l].copy_(torch.as_tensor(st[f"W{l}"]))
+            self.b[l].copy_(torch.as_tensor(st[f"b{l}"]))
+            self.Wb[l].copy_(self.W[l].to(torch.bfloat16))
+
+
+@dataclass
+class Norm:
+    """Feature standardisation and target scales."""
+    mu: np.ndarray
+    isd: np.ndarray
+    s_pos: float  # metres per unit of the x/y residual outputs
+    s_ang: float  # radians per unit of the angle outputs
+    s_ph: float   # radians per unit of the phase output
+    e_pos: float = 0.0
+    e_ang: float = 0.0
+
+    def __post_init__(self):
+        if self.e_pos == 0.0:
+            self.e_pos = self.s_pos
+        if self.e_ang == 0.0:
+            self.e_ang = self.s_ang
+
+
+@dataclass
+class Dataset:
+    feat: torch.Tensor     # [N, 48] fp32
+    target: torch.Tensor   # [N, 5] fp32, normalised residual (x, y, theta, phi, phase)
+    resid: torch.Tensor    # [5, N] fp64 raw residual (m, m, rad, rad, rad)
+    info: dict = field(default_factory=dict)
+
+
+def straight(field4, grid, s0, n=None):
+    """Straight-line estimate base [5, n] (fp64) and features [n, 48] (fp32)."""
+    lib = _lib.load()
+    n = s0.shape[1] if n is None else n
+    ld = s0.stride(0)
+    dev = s0.device
+    base = torch.empty((5, n), dtype=torch.float64, device=dev)
+    feat = torch.empty((n, NFEAT), dtype=torch.float32, device=dev)
+    lib.tt_straight_features(_p(field4), C.byref(grid), _p(s0), ld, n, _p(feat), NFEAT, _p(base), n,
+                             _lib.stream_ptr())
+    return base, feat
+
+
+def residual(rf, phase, base):
+    """[5, n] fp64 residual full - estimate, in (x, y, theta, phi, phase) order."""
+    return torch.stack([rf[0] - base[0], rf[2] - base[2], rf[1] - base[1], rf[3] - base[3], phase - base[4]])
+
+
+def build_dataset(n_fields, rays_per_field, cfg=None, seed=1000, device="cuda", log=None):
+    """Rays traced by the tracer through ``n_fields`` seeded fields: features
+    of their straight-line estimate and the residual of the full trace."""
+    cfg = cfg or Config()
+    cfg = Config(**{**cfg.__dict__, "n_rays": rays_per_field})
+    p = Pipeline(cfg, device)
+    feats, resids = [], []
+    for f in range(n_fields):
+        s = seed + 2 * f
+        p.generate_field(s)
+        p.gradient()
+        p.beam(s + 1)
+        p.sort()
+        p.trace()
+        base, feat = straight(p.field4, p.grid, p.s0)
+        feats.append(feat)
+        resids.append(residual(p.rf, p.phase, base))
+        if log and (f + 1) % max(1, n_fields // 8) == 0:
+            log(f"dataset: field {f + 1}/{n_fields}")
+    feat = torch.cat(feats)
+    resid = torch.cat(resids, dim=1)
+    return feat, resid, p
+
+
+def make_norm(feat, resid):
+    f = feat.double()
+    mu = f.mean(0)
+    sd = f.std(0)
+    isd = torch.where(sd > 0, 1.0 / sd, torch.zeros_like(sd))
+    r = resid.double()
+    ok = torch.isfinite(r).all(0)
+    r = r[:, ok]
+    s_pos = float(r[0:2].std()) or 1e-9
+    s_ang = float(r[2:4].std()) or 1e-12
+    s_ph = float(r[4].std()) or 1e-9
+    return Norm(mu.float().cpu().numpy(), isd.float().cpu().numpy(), s_pos, s_ang, s_ph)
+
+
+def normalise_targets(resid, norm):
+    s = torch.tensor([norm.s_pos, norm.s_pos, norm.s_ang, norm.s_ang, norm.s_ph], dtype=torch.float64,
+                     device=resid.device)
+    t = (resid / s[:, None]).T.float().contiguous()
+    return torch.nan_to_num(t, nan=0.0, posinf=0.0, neginf=0.0)
+
+
+class Surrogate:
+    def __init__(self, norm, mlp=None, device="cuda", seed=0):
+        self.norm = norm
+        self.device = device
+        self.mlp = mlp or MLP(device=device, seed=seed)
+        self.lib = _lib.load()
+        self.mu = torch.as_tensor(norm.mu, device=device)
+        self.isd = torch.as_tensor(norm.isd, device=device)
+        self.w_reg = torch.tensor([1.0, 1.0, 1.0, 1.0, 0.5], dtype=torch.float32, device=device)
+        self.err_lo = torch.tensor([0, 2], dtype=torch.int32, device=device)
+        self.err_hi = torch.tensor([2, 4], dtype=torch.int32, device=device)
+
+    def standardize(self, feat):
+        n = feat.shape[0]
+        X = torch.empty((n, NFEAT), dtype=torch.bfloat16, device=self.device)
+        self.lib.tt_standardize_bf16(_p(feat), NFEAT, n, NFEAT, _p(self.mu), _p(self.isd), _p(X), NFEAT,
+                                     _lib.stream_ptr())
+        return X
+
+    def train(self, feat, target, epochs=4, batch=8192, lr=2e-3, lr_min=5e-5, wd=0.0, log=None, seed=0):
+        """Adam with cosine decay on (features, normalised targets)."""
+        N = feat.shape[0]
+        X = self.standardize(feat)
+        gen = torch.Generator(device=self.device).manual_seed(seed)
+        steps = epochs * (N // batch)
+        loss = torch.zeros(2, dtype=torch.float64, device=self.device)
+        dY = torch.empty((batch, NOUT), dtype=torch.bfloat16, device=self.device)
+        Y = torch.empty((batch, NOUT), dtype=torch.float32, device=self.device)
+        hist = []
+        st = 0
+        for ep in range(epochs):
+            perm = torch.randperm(N, generator=gen, device=self.device)
+            loss.zero_()
+            for b in range(N // batch):
+                idx = perm[b * batch:(b + 1) * batch]
+                xb = X.index_select(0, idx)
+                tb = target.index_select(0, idx)
+                self.mlp.forward(xb, out=Y, keep=True)
+                self.lib.tt_surrogate_loss(_p(Y), NOUT, _p(tb), N_REG, _p(dY), NOUT, batch, N_REG, _p(self.w_reg),
+                                           N_ERR, _p(self.err_lo), _p(self.err_hi), NOUT, _p(loss),
+                                           _lib.stream_ptr())
+                self.mlp.backward(dY)
+                lr_t = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * st / max(1, steps)))
+                self.mlp.adam(lr_t, wd=wd)
+                st += 1
+            l = (loss / (N // batch * batch)).cpu().numpy()
+            hist.append(l)
+            if log:
+                log(f"epoch {ep + 1}/{epochs}: mse {l[0]:.4g}  err-head L1 {l[1]:.4g}")
+        return hist
+
+    def predict(self, feat, chunk=1 << 21):
+        n = feat.shape[0]
+        Y = torch.empty((n, NOUT), dtype=torch.float32, device=self.device)
+        for i in range(0, n, chunk):
+            j = min(n, i + chunk)
+            X = self.standardize(feat[i:j])
+            self.mlp.forward(X, out=Y[i:j])
+        return Y
+
+    def apply(self, base, Y, t_pos=1e-6, t_ang=1e-6, want_list=True):
+        n = base.shape[1]
+        rf = torch.empty((4, n), dtype=torch.float64, device=self.device)
+        phase = torch.empty(n, dtype=torch.float64, device=self.device)
+        flag = torch.empty(n, dtype=torch.int32, device=self.device)
+        lst = torch.empty(n, dtype=torch.int32, device=self.device) if want_list else None
+        cnt = torch.zeros(1, dtype=torch.int32, device=self.device)
+        sc = (C.c_float * 5)(self.norm.s_pos, self.norm.s_ang, self.norm.s_ph, self.norm.e_pos, self.norm.e_ang)
+        th = (C.c_double * 2)(t_pos, t_ang)
+        self.lib.tt_surrogate_apply(_p(base), n, _p(Y), NOUT, n, sc, th, _p(rf), n, _p(phase), _p(flag), _p(lst),
+                                    _p(cnt), _lib.stream_ptr())
+        return rf, phase, flag, lst, cnt
+
+    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, timings=None):
+        """Surrogate + fallback trace of rays s0 (6 x n, device): returns rf
+        (4 x n), phase (n) and the flag per ray (1 = re-traced)."""
+        n = s0.shape[1]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)] if timings is not None else None
+        if ev:
+            ev[0].record()
+        base, feat = straight(field4, grid, s0)
+        if ev:
+            ev[1].record()
+        Y = self.predict(feat)
+        if ev:
+            ev[2].record()
+        rf, phase, flag, lst, cnt = self.apply(base, Y, t_pos, t_ang)
+        nf = int(cnt.item())
+        if ev:
+            ev[3].record()
+        if nf > 0:
+            opts = _lib.TraceOpts(h, 0.0, 0)
+            self.lib.tt_trace_rk4(_p(field4), C.byref(grid), C.byref(opts), _p(s0), s0.stride(0), nf, _p(lst),
+                                  _p(rf), n, _p(phase), None, n, None, _lib.stream_ptr())
+        if ev:
+            ev[4].record()
+            torch.cuda.synchronize()
+            timings.update(straight=ev[0].elapsed_time(ev[1]), mlp=ev[1].elapsed_time(ev[2]),
+                           apply=ev[2].elapsed_time(ev[3]), fallback=ev[3].elapsed_time(ev[4]), n_fallback=nf)
+        return rf, phase, flag
+
+    def save(self, path):
+        np.savez(path, **self.mlp.state(), mu=self.norm.mu, isd=self.norm.isd,
+                 scales=np.array([self.norm.s_pos, self.norm.s_ang, self.norm.s_ph, self.norm.e_pos,
+                                  self.norm.e_ang]))
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        st = np.load(path)
+        sc = st["scales"]
+        norm = Norm(st["mu"], st["isd"], float(sc[0]), float(sc[1]), float(sc[2]), float(sc[3]), float(sc[4]))
+        s = cls(norm, device=device)
+        s.mlp.load_state(st)
+        return s
+
+
+def errors(rf, phase, ref_rf, ref_phase):
+    """Per-ray |error|: position (m, max of x, y), angle (rad, max of theta,
+    phi), phase (rad)."""
+    ep = torch.maximum((rf[0] - ref_rf[0]).abs(), (rf[2] - ref_rf[2]).abs())
+    ea = torch.maximum((rf[1] - ref_rf[1]).abs(), (rf[3] - ref_rf[3]).abs())
+    eph = (phase - ref_phase).abs()
+    return ep, ea, eph
+
+
+def summarize(e):
+    e = e[torch.isfinite(e)].double()
+    if e.numel() == 0:
+        return {}
+    q = torch.quantile(e[torch.randperm(e.numel(), device=e.device)[:1 << 20]], torch.tensor(
+        [0.5, 0.99, 0.999], dtype=torch.float64, device=e.device)).cpu().numpy()
+    return {"rms": float(e.pow(2).mean().sqrt()), "p50": float(q[0]), "p99": float(q[1]), "p999": float(q[2]),
+            "max": float(e.max())}
