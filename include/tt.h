@@ -196,18 +196,37 @@ int tt_detect(const tt_detector *dets, int32_t ndet, const double *rf, int64_t l
 
 /* -------------------------------------------------------------- surrogate
  * The learned correction.  None of these has a counterpart in the reference
- * (it has no surrogate); they serve the cheap straight-line estimate, the
- * MLP that corrects it, and its training. */
+ * (it has no surrogate); they serve the cheap path estimate, the MLP that
+ * corrects it, and its training. */
 
-/* Straight-line (second-order Born) estimate of each ray's exit state and its
- * 48 surrogate features.  base: 5 x ldb float64 [x, theta, y, phi, phase];
- * feat: n rows of ldf float32 (48 used): 16 z-segment sums of Gx, 16 of Gy,
- * then S_x, S_y, M_x, M_y, T_x, T_y, R_x, R_y, S_z, phase, U0, V0, W0 - 1,
- * entry-cell fractions x, y, and the path length in cells. */
-int tt_straight_features(const float *field4, const tt_grid *grid, const double *s0, int64_t ld, int64_t n,
-                         float *feat, int64_t ldf, double *base, int64_t ldb, void *stream);
+/* Straight-line z-block integrals of the node field for tt_path_features:
+ * block b covers cells [b k, min((b + 1) k, nz - 1)); per node column,
+ * G4[b] = sum over its cells of the cell's mid-plane value (the mean of the
+ * cell's two node planes, the exact z-integral of the trilinear field) and
+ * M4[b] = the exact first moment about the block centre,
+ * sum_c (c + 1/2 - zmid) mid_c + (f[b1] - f[b0]) / 12.  Both
+ * ceil((nz - 1) / k) * ny * nx float4. */
+int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t k, float *G4, float *M4, void *stream);
 
-/* Apply the MLP output Y (n rows, ld ldy >= 7, fp32) to the straight-line
+/* Cheap path estimate of each ray's exit state and its 48 surrogate
+ * features: one drift-kick-drift step per z-block of k cells (the block
+ * integral G4 at the mid-block position, plus v . grad(M4) for the ray's
+ * transverse drift in the block; the exit position gets -s M4_x / W for a
+ * kick spread over the block), against the eight field evaluations per z-cell
+ * of tt_trace_rk4.  A ray entering inside the grid takes single-cell steps
+ * from field4 up to the next block boundary.  order (nullable) as in
+ * tt_trace_rk4.  base: 5 x ldb float64 [x, theta, y, phi, phase]; feat: n rows
+ * of ldf >= 48 float32: 16 z-segment sums of the x kicks, 16 of the y kicks,
+ * then U - U0, V - V0, W - W0, the exit displacement from the entry line (x,
+ * y, cells), the phase, U0, V0, W0 - 1, the entry-cell fractions x, y, the x
+ * and y column changes between steps, the summed second-order proxies
+ * (dGx/dx Gx + dGx/dy Gy) / W^2 and (dGy/dx Gx + dGy/dy Gy) / W^2, and the
+ * path length in cells. */
+int tt_path_features(const float *field4, const float *G4, const float *M4, int32_t k, const tt_grid *grid,
+                     const double *s0, int64_t ld, int64_t n, const int32_t *order, float *feat, int64_t ldf,
+                     double *base, int64_t ldb, void *stream);
+
+/* Apply the MLP output Y (n rows, ld ldy >= 7, fp32) to the cheap path
  * estimate: rf = base[0:4] + Y[:, {0, 2, 1, 3}] * {s_pos, s_ang, s_pos, s_ang}
  * and phase = base[4] + Y[:, 4] * s_ph, with scales = {s_pos, s_ang, s_ph,
  * e_pos, e_ang}.  A ray is flagged for the full tracer when
@@ -219,9 +238,21 @@ int tt_surrogate_apply(const double *base, int64_t ldb, const float *Y, int64_t 
                        const double *thresholds, double *rf, int64_t ldr, double *phase, int32_t *flag,
                        int32_t *list, int32_t *nflag, void *stream);
 
+/* Fused inference of the surrogate network nfeat -> 256 -> 256 -> 256 -> nout
+ * (GELU, tanh form, on the hidden layers; bf16 MFMA, fp32 accumulation):
+ * Y[r][o] for fp32 features feat[r * ldf + f] standardised as
+ * (x - mu[f]) * inv_sd[f] and rounded to bf16.  nfeat <= 64, nout <= 16.
+ * w_packed: bf16 fragments, layer by layer (tiles x chunks 16 x 2, 16 x 8,
+ * 16 x 8, 1 x 8), fragment (nt, c) = 64 lanes x 8 with lane l element j =
+ * W[16 nt + l%16][k], k = 32c + 8(l/16) + j for the first layer and
+ * 32c + 16(j/4) + 4(l/16) + j%4 for the others (zero beyond the layer's
+ * size); bias_packed: fp32 256 + 256 + 256 + 16.  Both 16-byte aligned. */
+int tt_mlp_infer(const float *feat, int64_t ldf, int64_t n, int32_t nfeat, const float *mu, const float *inv_sd,
+                 const void *w_packed, const float *bias_packed, float *Y, int64_t ldy, int32_t nout, void *stream);
+
 /* C[m][n] = act(sum_k A(m,k) B(k,n) + bias[n]) on bf16 MFMA with fp32
  * accumulation; A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] (bf16).
- * gelu: 1 applies GELU (erf form); Zpre (optional) keeps the pre-activation;
+ * gelu: 1 applies GELU (tanh form, 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))); Zpre (optional) keeps the pre-activation;
  * outputs fp32 Cf and/or bf16 Cb with row stride ldc.  ksplit > 0 splits K
  * into chunks whose raw products are ADDED atomically into Cf (no bias,
  * activation or bf16 output). */

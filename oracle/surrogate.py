@@ -1,12 +1,12 @@
-"""FP64 oracle of the surrogate path: straight-line (Born) estimate and
+"""FP64 oracle of the surrogate path: the cheap path estimate and its
 features, and the MLP forward/backward/Adam.
 
 TEST INFRASTRUCTURE ONLY (see ``oracle/__init__``).  Nothing here follows a
 reference function: the reference has no surrogate.  Pins:
-  * ``straight_estimate`` -- the same midpoint-per-z-cell sums the GPU kernel
-    forms; pinned by exactness cases (null field; uniform slab: first order
-    is exact for the phase; linear slab: Born-1 deflection is exact) and
-    checked against the FP64 RK4 oracle for its order of accuracy.
+  * ``path_estimate`` -- the one-evaluation-per-z-cell drift-kick-drift
+    estimate the GPU kernel forms; pinned by exact cases (null field;
+    uniform slab phase; constant gradient: the mid-plane kick integrates the
+    parabola exactly) and by its agreement with the FP64 RK4 oracle.
   * ``mlp_forward`` / ``mlp_backward`` -- pinned by central finite
     differences (tests/test_oracle_surrogate.py) and by torch autograd in
     float64 on the CPU.
@@ -19,41 +19,72 @@ from .tracer import C, trilinear
 NSEG = 16
 
 
-def _eval_d(grid, px, py, iz, tz):
-    """Value of the 4 channels and d/dx, d/dy of channels x, y, w (per cell)."""
+def block_fields(grid, k):
+    """Straight-line z-block integrals of the node field, FP64, in the GPU's
+    per-cell units (G = gamma dz, Phi = k0 dz eta).  Block b covers cells
+    [b k, min((b + 1) k, ncell)); with m_c the mid-plane value of cell c (the
+    mean of its two node planes, i.e. the exact z-integral of the trilinear
+    field over the cell), returns G[b] = sum_c m_c and the exact first moment
+    M[b] = int (z - zmid_b) f dz = sum_c (c + 1/2 - zmid_b) m_c
+    + (f[b1] - f[b0]) / 12, zmid_b the block's centre, as arrays
+    [4, nx, ny, nb]."""
     nx, ny, nz = grid.shape
-    inside = (px >= 0) & (px <= nx - 1) & (py >= 0) & (py <= ny - 1)
-    fx = np.where(inside, px, 0.0)
-    fy = np.where(inside, py, 0.0)
-    ix = np.minimum(np.floor(fx).astype(np.int64), nx - 2)
-    iy = np.minimum(np.floor(fy).astype(np.int64), ny - 2)
-    tx, ty = fx - ix, fy - iy
-    out = []
-    dgx, dgy = [], []
-    for ch, v in enumerate(grid.fields):
-        c = {}
-        for a in (0, 1):
-            for b in (0, 1):
-                q0 = v[ix + a, iy + b, iz]
-                q1 = v[ix + a, iy + b, iz + 1]
-                c[a, b] = q0 + tz * (q1 - q0)
-        r0 = c[0, 0] + tx * (c[1, 0] - c[0, 0])
-        r1 = c[0, 1] + tx * (c[1, 1] - c[0, 1])
-        out.append(np.where(inside, r0 + ty * (r1 - r0), 0.0))
-        if ch in (0, 1, 3):
-            d_x = (c[1, 0] - c[0, 0]) + ty * ((c[1, 1] - c[0, 1]) - (c[1, 0] - c[0, 0]))
-            dgx.append(np.where(inside, d_x, 0.0))
-            dgy.append(np.where(inside, r1 - r0, 0.0))
-    return out, dgx, dgy
+    dz = grid.spacing[2]
+    scale = np.array([dz, dz, dz, grid.k0 * dz])
+    f = np.stack([v * s for v, s in zip(grid.fields, scale)])
+    mid = 0.5 * (f[..., :-1] + f[..., 1:])
+    ncell = nz - 1
+    nb = -(-ncell // k)
+    G = np.zeros((4, nx, ny, nb))
+    M = np.zeros((4, nx, ny, nb))
+    for bb in range(nb):
+        b0, b1 = bb * k, min(ncell, (bb + 1) * k)
+        zc = np.arange(b0, b1) + 0.5 - 0.5 * (b0 + b1)
+        G[..., bb] = mid[..., b0:b1].sum(-1)
+        # plus each cell's own moment: its field is linear in z, slope
+        # f[c + 1] - f[c], so int (z - z_c) f dz = (f[c + 1] - f[c]) / 12,
+        # which telescopes over the block
+        M[..., bb] = (mid[..., b0:b1] * zc).sum(-1) + (f[..., b1] - f[..., b0]) / 12.0
+    return G, M
 
 
-def straight_estimate(grid, s0):
-    """Second-order straight-line estimate and the 48 features, FP64.
+def _bilinear(A, mx, my, inside):
+    """Bilinear value and x / y slopes (per cell) of A [4, nx, ny] at cell
+    positions (mx, my); zero outside."""
+    nx, ny = A.shape[1:3]
+    ix = np.where(inside, np.minimum(np.floor(np.where(inside, mx, 0)).astype(np.int64), nx - 2), 0)
+    iy = np.where(inside, np.minimum(np.floor(np.where(inside, my, 0)).astype(np.int64), ny - 2), 0)
+    tx, ty = mx - ix, my - iy
+    a00, a10, a01, a11 = A[:, ix, iy], A[:, ix + 1, iy], A[:, ix, iy + 1], A[:, ix + 1, iy + 1]
+    v = (a00 * (1 - tx) + a10 * tx) * (1 - ty) + (a01 * (1 - tx) + a11 * tx) * ty
+    ddx = (a10 - a00) * (1 - ty) + (a11 - a01) * ty
+    ddy = (a01 - a00) * (1 - tx) + (a11 - a10) * tx
+    z = np.where(inside, 1.0, 0.0)
+    return v * z, ddx * z, ddy * z, ix, iy
 
-    ``grid`` is an oracle.tracer.NodeGrid (fields in the oracle's units:
-    gamma in 1/m and eta); the per-z-cell sums use G = gamma dz and
-    Phi = k0 dz eta like the GPU node field.  Returns base (5 x N: x, theta,
-    y, phi, phase) and feat (N x 48)."""
+
+def path_estimate(grid, s0, k=1, moments=True):
+    """Cheap path estimate and the 48 features, FP64.
+
+    The ray is advanced by drift-kick-drift steps over z-blocks of k cells:
+    drift half a block along the current direction, kick by the block's
+    straight-line field integral G at that position, drift the second half.
+    Two first-moment corrections make the straight-line integral follow the
+    ray: the field integral gets v . grad(M) (the ray's transverse drift
+    inside the block, v in cells per cell) and the position gets
+    -s M_x / W (a kick spread over the block rather than at its centre).
+    A ray entering inside the grid first takes single-cell steps from the node
+    field (the first cell partial) up to the next block boundary.
+
+    ``grid`` is an oracle.tracer.NodeGrid.  Returns base (5 x N: x, theta, y,
+    phi, phase) and feat (N x 48): 16 z-segment sums of the x and of the y
+    kicks (a step belongs to the segment of its first cell, q = 16 c / ncell),
+    then U - U0, V - V0, W - W0, the exit displacement from the entry line
+    (x, y, cells), the phase, U0, V0, W0 - 1, the entry-cell fractions x and
+    y, the x and y column changes between steps, the summed second-order
+    proxies q_x = (dGx/dx Gx + dGx/dy Gy) / W^2 and q_y likewise, and the
+    path length in cells.  ``moments=False`` drops both first-moment
+    corrections (a check that they matter, for the tests)."""
     nx, ny, nz = grid.shape
     x0, y0, z0 = grid.origin
     dx, dy, dz = grid.spacing
@@ -62,74 +93,116 @@ def straight_estimate(grid, s0):
     N = s0.shape[1]
     U0, V0, W0 = s0[3] / C, s0[4] / C, s0[5] / C
     sx, sy = dz / dx, dz / dy
-    slx, sly = sx * U0 / W0, sy * V0 / W0
     px0 = (s0[0] - x0) / dx
     py0 = (s0[1] - y0) / dy
     Zs = (s0[2] - z0) / dz
     pre = Zs < 0
-    px0 = np.where(pre, px0 + slx * -Zs, px0)
-    py0 = np.where(pre, py0 + sly * -Zs, py0)
+    px0 = np.where(pre, px0 + sx * U0 / W0 * -Zs, px0)
+    py0 = np.where(pre, py0 + sy * V0 / W0 * -Zs, py0)
     Zs = np.where(pre, 0.0, Zs)
-    Zend = float(nz - 1)
-    iW0 = 1.0 / W0
-    sp = np.sqrt(U0 ** 2 + V0 ** 2 + W0 ** 2) / W0
-    fsx, fsy = sx * iW0 ** 2, sy * iW0 ** 2
-    scale = np.array([dz, dz, dz, k0 * dz])
-    Sx, Sy, Sz, Mx, My, P = (np.zeros(N) for _ in range(6))
-    Tx, Ty, Rx, Ry, Dx, Dy = (np.zeros(N) for _ in range(6))
-    seg = np.zeros((N, 2 * NSEG))
     ncell = nz - 1
-    for iz in range(ncell):
-        act = Zs < iz + 1
-        zl = np.maximum(Zs, iz)
-        w = np.where(act, (iz + 1) - zl, 0.0)
-        zm = zl + 0.5 * w
-        px = px0 + slx * (zm - Zs)
-        py = py0 + sly * (zm - Zs)
-        vals, dgx, dgy = _eval_d(grid, px, py, iz, zm - iz)
-        g = [v * s for v, s in zip(vals, scale)]
-        dgx = [d * s for d, s in zip(dgx, (dz, dz, k0 * dz))]
-        dgy = [d * s for d, s in zip(dgy, (dz, dz, k0 * dz))]
-        h = 0.5 * w
-        d1x = fsx * (Dx + Sx * h + 0.5 * g[0] * h * h)
-        d1y = fsy * (Dy + Sy * h + 0.5 * g[1] * h * h)
-        f2x = dgx[0] * d1x + dgy[0] * d1y
-        f2y = dgx[1] * d1x + dgy[1] * d1y
-        lever = Zend - zm
-        Tx += w * f2x
-        Ty += w * f2y
-        Rx += w * lever * f2x
-        Ry += w * lever * f2y
-        Mx += w * lever * g[0]
-        My += w * lever * g[1]
-        Dx += Sx * w + 0.5 * g[0] * w * w
-        Dy += Sy * w + 0.5 * g[1] * w * w
-        Sx += w * g[0]
-        Sy += w * g[1]
-        Sz += w * g[2]
-        P += w * sp * (g[3] + dgx[2] * d1x + dgy[2] * d1y)
-        q = (iz * NSEG) // ncell
-        seg[:, q] += w * g[0]
-        seg[:, NSEG + q] += w * g[1]
-    Ue = U0 + (Sx + Tx) * iW0
-    Ve = V0 + (Sy + Ty) * iW0
-    We = W0 + Sz * iW0
-    L = Zend - Zs
-    pxe = px0 + slx * L + fsx * (Mx + Rx)
-    pye = py0 + sly * L + fsy * (My + Ry)
-    base = np.stack([x0 + pxe * dx, np.arctan(Ue / We), y0 + pye * dy, np.arctan(Ve / We), P])
-    glob = np.stack([Sx, Sy, Mx, My, Tx, Ty, Rx, Ry, Sz, P, U0, V0, W0 - 1.0,
-                     px0 - np.floor(px0), py0 - np.floor(py0), Zend - Zs], axis=1)
+    G, M = block_fields(grid, k)
+    scale = np.array([dz, dz, dz, k0 * dz])
+    st = dict(px=px0.copy(), py=py0.copy(), U=U0.copy(), V=V0.copy(), W=W0.copy(), ph=np.zeros(N),
+              cx=np.zeros(N), cy=np.zeros(N), qx=np.zeros(N), qy=np.zeros(N), cix=np.full(N, -1),
+              ciy=np.full(N, -1))
+    seg = np.zeros((N, 2 * NSEG))
+    rows = np.arange(N)
+
+    def step(act, c0, w, sample):
+        """One drift-kick-drift step of width w (cells) starting at cell c0
+        for the rays in act; sample(mx, my, inside) -> (g, mval, q)."""
+        U, V, W = st["U"], st["V"], st["W"]
+        hm = 0.5 * w
+        mx = st["px"] + hm * sx * U / W
+        my = st["py"] + hm * sy * V / W
+        inside = (mx >= 0) & (mx <= nx - 1) & (my >= 0) & (my <= ny - 1)
+        g, mval, qx, qy, ix, iy = sample(mx, my, inside)
+        sp = np.sqrt(U * U + V * V + W * W) / W
+        iWm = 1.0 / (W + hm * g[2] / W)
+        Un, Vn, Wn = U + g[0] * iWm, V + g[1] * iWm, W + g[2] * iWm
+        pxn = mx + hm * sx * Un / Wn - sx * mval[0] / Wn
+        pyn = my + hm * sy * Vn / Wn - sy * mval[1] / Wn
+        upd = lambda key, val: st.__setitem__(key, np.where(act, val, st[key]))
+        first = st["cix"] < 0
+        upd("cx", st["cx"] + np.where(~first & (ix != st["cix"]), 1.0, 0.0))
+        upd("cy", st["cy"] + np.where(~first & (iy != st["ciy"]), 1.0, 0.0))
+        upd("qx", st["qx"] + qx)
+        upd("qy", st["qy"] + qy)
+        upd("cix", ix)
+        upd("ciy", iy)
+        upd("ph", st["ph"] + sp * g[3])
+        upd("U", Un)
+        upd("V", Vn)
+        upd("W", Wn)
+        upd("px", pxn)
+        upd("py", pyn)
+        q = (c0 * NSEG) // ncell
+        q = np.minimum(np.where(act, q, 0), NSEG - 1)
+        np.add.at(seg, (rows[act], q[act]), g[0][act])
+        np.add.at(seg, (rows[act], NSEG + q[act]), g[1][act])
+
+    # single-cell steps from the entry cell to the next block boundary
+    iz0 = np.where(Zs >= ncell, ncell, np.floor(Zs)).astype(np.int64)
+    on_boundary = (Zs == iz0) & (iz0 % k == 0)
+    bstart = np.where(on_boundary, iz0, np.minimum(ncell, (iz0 // k + 1) * k))
+    for c in range(ncell):
+        act = (c >= iz0) & (c < bstart)
+        if not act.any():
+            continue
+        w = np.where(c == iz0, (c + 1) - Zs, 1.0)
+        tz = 1.0 - 0.5 * w
+
+        def cell_sample(mx, my, inside, c=c, tz=tz):
+            vals = trilinear(grid.fields, grid.origin, grid.spacing, grid.shape, np.where(inside, mx, -1.0),
+                             np.where(inside, my, -1.0), c + tz)
+            g = np.stack([v * s for v, s in zip(vals, scale)]) * w
+            ix = np.where(inside, np.minimum(np.floor(np.where(inside, mx, 0)).astype(np.int64), nx - 2), 0)
+            iy = np.where(inside, np.minimum(np.floor(np.where(inside, my, 0)).astype(np.int64), ny - 2), 0)
+            return g, np.zeros_like(g), 0.0, 0.0, ix, iy
+        step(act, c, w, cell_sample)
+    # blocks
+    nb = G.shape[-1]
+    for bb in range(nb):
+        b0, b1 = bb * k, min(ncell, (bb + 1) * k)
+        act = bstart <= b0
+        if not act.any():
+            continue
+        w = float(b1 - b0)
+
+        def block_sample(mx, my, inside, bb=bb):
+            Gv, Gdx, Gdy, ix, iy = _bilinear(G[..., bb], mx, my, inside)
+            Mv, Mdx, Mdy, _, _ = _bilinear(M[..., bb], mx, my, inside)
+            U, V, W = st["U"], st["V"], st["W"]
+            vx, vy = sx * U / W, sy * V / W
+            g = Gv + vx * Mdx + vy * Mdy if moments else Gv
+            iW2 = 1.0 / (W * W)
+            qx = (Gdx[0] * Gv[0] + Gdy[0] * Gv[1]) * iW2
+            qy = (Gdx[1] * Gv[0] + Gdy[1] * Gv[1]) * iW2
+            return g, (Mv if moments else 0 * Mv), qx, qy, ix, iy
+        step(act, b0, w, block_sample)
+    px, py, U, V, W, ph = st["px"], st["py"], st["U"], st["V"], st["W"], st["ph"]
+    L = ncell - Zs
+    base = np.stack([x0 + px * dx, np.arctan(U / W), y0 + py * dy, np.arctan(V / W), ph])
+    glob = np.stack([U - U0, V - V0, W - W0, px - (px0 + sx * U0 / W0 * L), py - (py0 + sy * V0 / W0 * L), ph,
+                     U0, V0, W0 - 1.0, px0 - np.floor(px0), py0 - np.floor(py0), st["cx"], st["cy"], st["qx"],
+                     st["qy"], L], axis=1)
     return base, np.concatenate([seg, glob], axis=1)
 
 
 # --------------------------------------------------------------------- MLP
+GELU_K0 = np.sqrt(2.0 / np.pi)
+GELU_K1 = 0.044715
+
+
 def gelu(x):
-    return 0.5 * x * (1.0 + erf(x / np.sqrt(2.0)))
+    """GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3)))."""
+    return 0.5 * x * (1.0 + np.tanh(GELU_K0 * (x + GELU_K1 * x ** 3)))
 
 
 def gelu_grad(x):
-    return 0.5 * (1.0 + erf(x / np.sqrt(2.0))) + x * np.exp(-0.5 * x * x) / np.sqrt(2 * np.pi)
+    t = np.tanh(GELU_K0 * (x + GELU_K1 * x ** 3))
+    return 0.5 * (1.0 + t) + 0.5 * x * (1.0 - t * t) * GELU_K0 * (1.0 + 3.0 * GELU_K1 * x * x)
 
 
 def mlp_forward(params, X):

@@ -82,13 +82,15 @@ _SIGNATURES = {
     "tt_ray_sort": [P, I64, I64, C.POINTER(Grid), I32, P, P, P],
     "tt_optics_apply": [C.POINTER(Train), P, I64, I64, P, I64, P],
     "tt_detect": [C.POINTER(Detector), I32, P, I64, P, I64, P],
-    "tt_straight_features": [P, C.POINTER(Grid), P, I64, I64, P, I64, P, I64, P],
+    "tt_field_blocks": [P, C.POINTER(Grid), I32, P, P, P],
+    "tt_path_features": [P, P, P, I32, C.POINTER(Grid), P, I64, I64, P, P, I64, P, I64, P],
     "tt_surrogate_apply": [P, I64, P, I64, I64, P, P, P, I64, P, P, P, P, P],
     "tt_gemm_bf16": [C.POINTER(GemmArgs), P],
     "tt_gelu_backward": [P, P, P, P, I64, P],
     "tt_bias_grad": [P, I64, I32, I64, P, P],
     "tt_surrogate_loss": [P, I64, P, I64, P, I64, I64, I32, P, I32, P, P, I32, P, P],
     "tt_adam": [P, P, P, P, P, I64, D, D, D, D, I64, D, P],
+    "tt_mlp_infer": [P, I64, I64, I32, P, P, P, P, P, I64, I32, P],
     "tt_standardize_bf16": [P, I64, I64, I32, P, P, P, I64, P],
 }
 

@@ -1,8 +1,8 @@
 // Surrogate MLP kernels: BF16 MFMA GEMM with fused bias/GELU epilogue
 // (forward and both backward products), GELU backward, bias gradient, loss
 // gradient and Adam.  No reference counterpart: the reference has no learned
-// component; these serve the learned correction of the straight-line ray
-// estimate (tt_straight_features) against the full RK4 trace.
+// component; these serve the learned correction of the cheap path estimate
+// (tt_path_features) against the full RK4 trace.
 //
 // GEMM: C[m][n] = epi( sum_k A(m,k) B(k,n) + bias[n] ), A and B bf16 with
 // arbitrary element strides, fp32 accumulation on v_mfma_f32_16x16x32_bf16.
@@ -26,11 +26,20 @@ namespace {
 constexpr int BM = 64, BN = 64, BK = 32;
 constexpr int PADK = 8;  // LDS row padding (bf16 elements) to spread banks
 
-__device__ __forceinline__ float gelu_f(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
+// evaluated as x * s with s = 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u)) and
+// exp(-2u) = 2^(x (a + b x^2)), a = -2 sqrt(2/pi) log2(e), b = 0.044715 a.
+// Saturates correctly: s -> 0 (exp2 -> inf) and s -> 1 (exp2 -> 0).
+__device__ __forceinline__ float gelu_s(float x) {
+    const float e = __builtin_amdgcn_exp2f(x * fmaf(-0.1029432395800235f, x * x, -2.302208198144325f));
+    return __builtin_amdgcn_rcpf(1.0f + e);
+}
+__device__ __forceinline__ float gelu_f(float x) { return x * gelu_s(x); }
+// d/dx [x s(2u)] = s + x 2 s (1 - s) u', u' = sqrt(2/pi) (1 + 3 * 0.044715 x^2)
 __device__ __forceinline__ float gelu_grad(float x) {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    const float pdf = 0.39894228040143268f * __expf(-0.5f * x * x);
-    return cdf + x * pdf;
+    const float s = gelu_s(x);
+    const float du = 0.7978845608028654f * fmaf(0.134145f, x * x, 1.0f);
+    return fmaf(2.0f * x * s * (1.0f - s), du, s);
 }
 
 // Fill a [64 rows][32 k] LDS tile of operand X(row, k) = X[row*srow + k*sk].
@@ -238,6 +247,124 @@ __global__ void standardize_kernel(const float *x, int64_t ldx, int64_t rows, in
     out[i] = (__bf16)v;
 }
 
+// ------------------------------------------------------------ inference
+// Fused forward pass of the surrogate network in[<=64] -> 256 -> 256 -> 256
+// -> out[<=16] (GELU on the hidden layers), input standardisation included.
+// Computed transposed, H'^T = W H^T: the weights are the MFMA A operand and
+// the activations of 16 rays the B operand, so a layer's D tiles (lane l:
+// feature 4(l/16)+i of a 16-feature tile, ray l%16) become the next layer's
+// B operand in registers with no data movement: the 8 bf16 of lane l for
+// k-chunk c are features 32c + 16(j/4) + 4(l/16) + j%4, and the next layer's
+// weight columns are packed in that order (kperm) on the host.  The first
+// layer reads features 32c + 8(l/16) + j directly.
+//
+// Packed weights: per layer, fragments (tile nt, chunk c) in order
+// nt * KC + c, each 64 lanes x 8 bf16 (lane l: W[16nt + l%16][kperm(c, l/16,
+// j)]), zero-padded.  Layer shapes (NT x KC): 16 x 2, 16 x 8, 16 x 8, 1 x 8.
+// A block of 4 waves takes 128 rays (32 per wave, two 16-ray B tiles) and
+// stages the fragments of up to 8 output tiles at a time through 64 KB of
+// LDS, so each fragment is read once from L2 per block and once from LDS per
+// wave for two MFMAs.
+constexpr int kInferRays = 128;
+constexpr int kLdsFrags = 64;  // 64 KB
+
+struct InferArgs {
+    const float *feat;
+    int64_t ldf, n;
+    int nfeat, nout;
+    const float *mu, *isd;
+    const __bf16 *w;
+    const float *bias;  // 256 + 256 + 256 + 16
+    float *Y;
+    int64_t ldy;
+};
+
+template <int NFR>
+__device__ __forceinline__ void stage_frags(const __bf16 *src, __bf16 *lds) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+    uint4 *d4 = reinterpret_cast<uint4 *>(lds);
+#pragma unroll
+    for (int i = 0; i < NFR * 64 / 256; ++i) d4[i * 256 + threadIdx.x] = s4[i * 256 + threadIdx.x];
+}
+
+// One hidden layer: x (KC chunks x 2 ray tiles) -> y (8 chunks x 2 tiles).
+template <int KC>
+__device__ __forceinline__ void hidden_layer(const __bf16 *w, const float *bias, __bf16 *lds, const bf16x8 (&x)[KC][2],
+                                             bf16x8 (&y)[8][2]) {
+    constexpr int NG = kLdsFrags / KC < 16 ? kLdsFrags / KC : 16;  // output tiles per LDS stage
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+#pragma unroll
+    for (int grp = 0; grp < 16 / NG; ++grp) {
+        __syncthreads();
+        stage_frags<NG * KC>(w + (size_t)grp * NG * KC * 512, lds);
+        __syncthreads();
+#pragma unroll
+        for (int tl = 0; tl < NG; ++tl) {
+            f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int c = 0; c < KC; ++c) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(lds + ((tl * KC + c) * 64 + lane) * 8);
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][0], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][1], acc1, 0, 0, 0);
+            }
+            const int nt = grp * NG + tl;
+            const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * nt + 4 * g);
+            const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                y[nt >> 1][0][4 * (nt & 1) + i] = (__bf16)gelu_f(acc0[i] + bv[i]);
+                y[nt >> 1][1][4 * (nt & 1) + i] = (__bf16)gelu_f(acc1[i] + bv[i]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs a) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds[kLdsFrags * 512];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const int64_t ray0 = (int64_t)blockIdx.x * kInferRays + wave * 32;
+    // input: lane holds features 32c + 8g + j of rays ray0 + 16t + r
+    bf16x8 x0[2][2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int64_t ray = ray0 + 16 * t + r;
+        const float *row = a.feat + ray * a.ldf;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int f = 32 * c + 8 * g + j;
+                float v = 0.f;
+                if (ray < a.n && f < a.nfeat) v = (row[f] - a.mu[f]) * a.isd[f];
+                x0[c][t][j] = (__bf16)v;
+            }
+    }
+    bf16x8 h1[8][2], h2[8][2];
+    hidden_layer<2>(a.w, a.bias, lds, x0, h1);
+    hidden_layer<8>(a.w + 16 * 2 * 512, a.bias + 256, lds, h1, h2);
+    hidden_layer<8>(a.w + (16 * 2 + 16 * 8) * 512, a.bias + 512, lds, h2, h1);
+    // output tile: 16 rows (nout used), K = 256
+    __syncthreads();
+    stage_frags<8>(a.w + (16 * 2 + 2 * 16 * 8) * 512, lds);
+    __syncthreads();
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const bf16x8 w = *reinterpret_cast<const bf16x8 *>(lds + (c * 64 + lane) * 8);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, h1[c][0], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, h1[c][1], acc1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int o = 4 * g + i;
+        if (o >= a.nout) continue;
+        const float b = a.bias[768 + o];
+        const int64_t ra = ray0 + r, rb = ray0 + 16 + r;
+        if (ra < a.n) a.Y[ra * a.ldy + o] = acc0[i] + b;
+        if (rb < a.n) a.Y[rb * a.ldy + o] = acc1[i] + b;
+    }
+}
+
 }  // namespace
 
 extern "C" int tt_gemm_bf16(const tt_gemm_args *g, void *stream) {
@@ -333,5 +460,34 @@ extern "C" int tt_standardize_bf16(const float *x, int64_t ldx, int64_t rows, in
     hipLaunchKernelGGL(standardize_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream),
                        x, ldx, rows, cols, mu, inv_sd, static_cast<__bf16 *>(out_bf16), ldo);
     TT_LAUNCH_CHECK("tt_standardize_bf16 launch");
+    return TT_OK;
+}
+
+extern "C" int tt_mlp_infer(const float *feat, int64_t ldf, int64_t n, int32_t nfeat, const float *mu,
+                            const float *inv_sd, const void *w_packed, const float *bias_packed, float *Y, int64_t ldy,
+                            int32_t nout, void *stream) {
+    TT_REQUIRE(feat && mu && inv_sd && w_packed && bias_packed && Y, "tt_mlp_infer: null pointer");
+    TT_REQUIRE(nfeat > 0 && nfeat <= 64 && ldf >= nfeat, "tt_mlp_infer: need 0 < nfeat <= 64 and ldf >= nfeat");
+    TT_REQUIRE(nout > 0 && nout <= 16 && ldy >= nout, "tt_mlp_infer: need 0 < nout <= 16 and ldy >= nout");
+    TT_REQUIRE(((uintptr_t)w_packed & 15) == 0 && ((uintptr_t)bias_packed & 15) == 0,
+               "tt_mlp_infer: packed weights and biases must be 16-byte aligned");
+    TT_REQUIRE(n >= 0, "tt_mlp_infer: n < 0");
+    if (n == 0) return TT_OK;
+    InferArgs a;
+    a.feat = feat;
+    a.ldf = ldf;
+    a.n = n;
+    a.nfeat = nfeat;
+    a.nout = nout;
+    a.mu = mu;
+    a.isd = inv_sd;
+    a.w = static_cast<const __bf16 *>(w_packed);
+    a.bias = bias_packed;
+    a.Y = Y;
+    a.ldy = ldy;
+    const int64_t blocks = tt::ceil_div(n, kInferRays);
+    TT_REQUIRE(blocks <= 0x7fffffff, "tt_mlp_infer: too many rays");
+    hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, tt::as_stream(stream), a);
+    TT_LAUNCH_CHECK("tt_mlp_infer launch");
     return TT_OK;
 }

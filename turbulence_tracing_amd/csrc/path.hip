@@ -1,0 +1,382 @@
+// Cheap path estimate of each ray and the surrogate features.
+//
+// Not a reference function: the cheap estimate whose residual against the
+// full trace (tt_trace_rk4, the work of particle_tracker.py:312-380) the
+// surrogate MLP learns.  The field is integrated along straight lines over
+// z-blocks of k cells: block_kernel stores, per block and node column, the
+// exact z-integral G of the trilinear field (the sum of the cells' mid-plane
+// values) and its first moment M about the block centre.  Each ray then takes
+// one drift-kick-drift step per block: drift half a block along its current
+// direction, kick by G at that position, drift the second half.  Two
+// first-moment terms make the straight-line integral follow the ray: the
+// kick gets v . grad(M) (v = the ray's transverse drift in cells per cell),
+// which is the integral of the field's linear variation along the tilted
+// path, and the position gets -s M_x / W, because the kick is spread over the
+// block instead of applied at its centre.  A ray entering inside the grid
+// first takes single-cell steps from the node field up to the next block
+// boundary.  The full RK4 takes 8 field evaluations per z-cell (two steps of
+// four stages); this takes one per k cells.
+//
+// Outputs per ray: base = [x, theta, y, phi, phase] (float64, the rf layout
+// plus phase) and 48 float32 features: 16 z-segment sums of the x kicks and of
+// the y kicks (a step belongs to the segment of its first cell, 16 c / ncell),
+// then U - U0, V - V0, W - W0, the exit displacement from the entry line (x,
+// y, cells), the phase, U0, V0, W0 - 1, the entry-cell fractions in x and y,
+// the numbers of column changes in x and y between steps, the summed
+// second-order proxies q_x = (dGx/dx Gx + dGx/dy Gy) / W^2 and q_y = (dGy/dx
+// Gx + dGy/dy Gy) / W^2 over the blocks, and the path length (cells).
+#include "tt_internal.h"
+
+namespace {
+
+constexpr int kSeg = 16;
+
+struct PathArgs {
+    const float4 *field;  // node field [nz][ny][nx]
+    const float4 *G, *M;  // block integrals and first moments [nb][ny][nx]
+    int k, nb;
+    int nx, ny, nz;
+    double x0, y0, z0, dx, dy, dz;
+    float sx, sy;
+    const double *s0;
+    int64_t ld, n;
+    const int32_t *order;
+    float *feat;
+    int64_t ldf;
+    double *base;  // 5 x ldb
+    int64_t ldb;
+};
+
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+    return make_float4(fmaf(t, b.x - a.x, a.x), fmaf(t, b.y - a.y, a.y), fmaf(t, b.z - a.z, a.z),
+                       fmaf(t, b.w - a.w, a.w));
+}
+__device__ __forceinline__ float4 sub4(float4 a, float4 b) {
+    return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+}
+__device__ __forceinline__ float4 bilerp(const float4 *c, float tx, float ty) {
+    return lerp4(lerp4(c[0], c[1], tx), lerp4(c[2], c[3], tx), ty);
+}
+__device__ __forceinline__ void load_corners(const float4 *p, int nx, float4 *c) {
+    c[0] = p[0];
+    c[1] = p[1];
+    c[2] = p[nx];
+    c[3] = p[nx + 1];
+}
+
+struct PathState {
+    float dpx, dpy, U, V, Wd;
+    double ph;
+    float segx, segy, crossx, crossy, qx, qy;
+    int cix, ciy;
+    int seg, seg_end;
+};
+
+// Mid-step position of a step of width w from the current state (cells),
+// its column (0 with inb = false outside the x/y range) and fractions.
+struct Mid {
+    float W, iW, dmx, dmy, tx, ty;
+    int ix, iy;
+    bool inb;
+};
+
+__device__ __forceinline__ Mid half_drift(const PathArgs &a, const PathState &S, float pxb, float pyb, float w) {
+    Mid m;
+    m.W = 1.0f + S.Wd;
+    m.iW = __builtin_amdgcn_rcpf(m.W);
+    const float hm = 0.5f * w;
+    m.dmx = fmaf(hm * a.sx * S.U, m.iW, S.dpx);
+    m.dmy = fmaf(hm * a.sy * S.V, m.iW, S.dpy);
+    const float mx = pxb + m.dmx, my = pyb + m.dmy;
+    m.inb = mx >= 0.f && mx <= (float)(a.nx - 1) && my >= 0.f && my <= (float)(a.ny - 1);
+    m.ix = m.inb ? min((int)mx, a.nx - 2) : 0;
+    m.iy = m.inb ? min((int)my, a.ny - 2) : 0;
+    m.tx = mx - (float)m.ix;
+    m.ty = my - (float)m.iy;
+    return m;
+}
+
+// Kick by g (the step's field integral), drift the second half with the
+// position moment (mvx, mvy) and accumulate the features of a step that
+// starts at cell c0.
+__device__ __forceinline__ void kick_drift(const PathArgs &a, const Mid &m, float4 g, float w, float mvx, float mvy,
+                                           int c0, int ncell, PathState &S, float *f) {
+    const float hm = 0.5f * w;
+    const float iWm = __builtin_amdgcn_rcpf(fmaf(hm * g.z, m.iW, m.W));
+    const float sp = __builtin_amdgcn_sqrtf(fmaf(S.U, S.U, fmaf(S.V, S.V, m.W * m.W))) * m.iW;
+    S.ph += (double)(sp * g.w);
+    S.U = fmaf(g.x, iWm, S.U);
+    S.V = fmaf(g.y, iWm, S.V);
+    S.Wd = fmaf(g.z, iWm, S.Wd);
+    const float iW1 = __builtin_amdgcn_rcpf(1.0f + S.Wd);
+    S.dpx = fmaf(fmaf(hm, S.U, -mvx) * a.sx, iW1, m.dmx);
+    S.dpy = fmaf(fmaf(hm, S.V, -mvy) * a.sy, iW1, m.dmy);
+    // z-segment sums: segment q holds cells [ceil(q ncell / 16), ceil((q + 1) ncell / 16))
+    while (c0 >= S.seg_end) {
+        f[S.seg] = S.segx;
+        f[kSeg + S.seg] = S.segy;
+        S.segx = S.segy = 0.f;
+        ++S.seg;
+        S.seg_end = ((S.seg + 1) * ncell + kSeg - 1) / kSeg;
+    }
+    S.segx += g.x;
+    S.segy += g.y;
+    if (S.cix >= 0) {
+        S.crossx += (m.ix != S.cix) ? 1.f : 0.f;
+        S.crossy += (m.iy != S.ciy) ? 1.f : 0.f;
+    }
+    S.cix = m.ix;
+    S.ciy = m.iy;
+}
+
+__global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int64_t r = a.order ? (int64_t)a.order[t] : t;
+    const double *s = a.s0 + r;
+    const double x = s[0], y = s[a.ld], z = s[2 * a.ld];
+    const double U0 = s[3 * a.ld] / tt::kC, V0 = s[4 * a.ld] / tt::kC, W0 = s[5 * a.ld] / tt::kC;
+    float *f = a.feat + r * a.ldf;
+    double *b = a.base + r;
+    const double nanv = __longlong_as_double(0x7ff8000000000000ll);
+    if (!(W0 > 0.0) || isnan(x) || isnan(y) || isnan(z) || isnan(U0) || isnan(V0)) {
+        for (int c = 0; c < 5; ++c) b[c * a.ldb] = nanv;
+        for (int c = 0; c < 48; ++c) f[c] = 0.f;
+        return;
+    }
+    double px0 = (x - a.x0) / a.dx, py0 = (y - a.y0) / a.dy, Zs = (z - a.z0) / a.dz;
+    if (Zs < 0.0) {  // zero field before the grid: straight to the first plane
+        px0 += (double)a.sx * U0 / W0 * -Zs;
+        py0 += (double)a.sy * V0 / W0 * -Zs;
+        Zs = 0.0;
+    }
+    const int ncell = a.nz - 1;
+    const float pxb = (float)px0, pyb = (float)py0;
+    PathState S;
+    S.dpx = S.dpy = 0.f;
+    S.U = (float)U0;
+    S.V = (float)V0;
+    S.Wd = (float)(W0 - 1.0);
+    S.ph = 0.0;
+    S.segx = S.segy = S.crossx = S.crossy = S.qx = S.qy = 0.f;
+    S.cix = S.ciy = -1;
+    for (int q = 0; q < 2 * kSeg; ++q) f[q] = 0.f;
+    const int iz0 = Zs >= (double)ncell ? ncell : (int)Zs;
+    S.seg = (iz0 * kSeg) / (ncell > 0 ? ncell : 1);
+    S.seg_end = ((S.seg + 1) * ncell + kSeg - 1) / kSeg;
+    const bool on_boundary = Zs == (double)iz0 && iz0 % a.k == 0;
+    const int bstart = on_boundary ? iz0 : min(ncell, (iz0 / a.k + 1) * a.k);
+    // single-cell steps from the node field: the first one may be partial,
+    // sampled at its mid-step plane 1 - w/2 between planes c and c + 1
+    for (int c = iz0; c < bstart; ++c) {
+        const float w = c == iz0 ? (float)((double)(c + 1) - Zs) : 1.0f;
+        const Mid m = half_drift(a, S, pxb, pyb, w);
+        const float4 *p = a.field + ((int64_t)c * a.ny + m.iy) * a.nx + m.ix;
+        float4 lo[4], hi[4];
+        load_corners(p, a.nx, lo);
+        load_corners(p + (int64_t)a.nx * a.ny, a.nx, hi);
+        const float tz = 1.0f - 0.5f * w;
+        for (int q = 0; q < 4; ++q) lo[q] = lerp4(lo[q], hi[q], tz);
+        float4 g = bilerp(lo, m.tx, m.ty);
+        g = m.inb ? make_float4(w * g.x, w * g.y, w * g.z, w * g.w) : make_float4(0.f, 0.f, 0.f, 0.f);
+        kick_drift(a, m, g, w, 0.f, 0.f, c, ncell, S, f);
+    }
+    // z-blocks
+    const int64_t bplane = (int64_t)a.nx * a.ny;
+    for (int bb = bstart / a.k; bb < a.nb && bstart < ncell; ++bb) {
+        const int b0 = bb * a.k;
+        const float w = (float)(min(ncell, b0 + a.k) - b0);
+        const Mid m = half_drift(a, S, pxb, pyb, w);
+        const int64_t o = bb * bplane + (int64_t)m.iy * a.nx + m.ix;
+        float4 Gc[4], Mc[4];
+        load_corners(a.G + o, a.nx, Gc);
+        load_corners(a.M + o, a.nx, Mc);
+        const float4 G = bilerp(Gc, m.tx, m.ty);
+        const float4 Mdx = lerp4(sub4(Mc[1], Mc[0]), sub4(Mc[3], Mc[2]), m.ty);
+        const float4 Mdy = lerp4(sub4(Mc[2], Mc[0]), sub4(Mc[3], Mc[1]), m.tx);
+        const float mvx = fmaf(m.ty, fmaf(m.tx, Mc[3].x - Mc[2].x, Mc[2].x) - fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x),
+                               fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x));
+        const float mvy = fmaf(m.ty, fmaf(m.tx, Mc[3].y - Mc[2].y, Mc[2].y) - fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y),
+                               fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y));
+        const float vx = a.sx * S.U * m.iW, vy = a.sy * S.V * m.iW;
+        float4 g = make_float4(fmaf(vy, Mdy.x, fmaf(vx, Mdx.x, G.x)), fmaf(vy, Mdy.y, fmaf(vx, Mdx.y, G.y)),
+                               fmaf(vy, Mdy.z, fmaf(vx, Mdx.z, G.z)), fmaf(vy, Mdy.w, fmaf(vx, Mdx.w, G.w)));
+        // second-order proxies from the slopes of G
+        const float Gdxx = fmaf(m.ty, (Gc[3].x - Gc[2].x) - (Gc[1].x - Gc[0].x), Gc[1].x - Gc[0].x);
+        const float Gdyx = fmaf(m.tx, (Gc[3].x - Gc[1].x) - (Gc[2].x - Gc[0].x), Gc[2].x - Gc[0].x);
+        const float Gdxy = fmaf(m.ty, (Gc[3].y - Gc[2].y) - (Gc[1].y - Gc[0].y), Gc[1].y - Gc[0].y);
+        const float Gdyy = fmaf(m.tx, (Gc[3].y - Gc[1].y) - (Gc[2].y - Gc[0].y), Gc[2].y - Gc[0].y);
+        const float iW2 = m.iW * m.iW;
+        if (m.inb) {
+            S.qx = fmaf(fmaf(Gdxx, G.x, Gdyx * G.y), iW2, S.qx);
+            S.qy = fmaf(fmaf(Gdxy, G.x, Gdyy * G.y), iW2, S.qy);
+        } else {
+            g = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        kick_drift(a, m, g, w, m.inb ? mvx : 0.f, m.inb ? mvy : 0.f, b0, ncell, S, f);
+    }
+    if (S.seg < kSeg) {
+        f[S.seg] = S.segx;
+        f[kSeg + S.seg] = S.segy;
+    }
+    const float U = S.U, V = S.V, Wd = S.Wd, dpx = S.dpx, dpy = S.dpy;
+    const double ph = S.ph;
+    const double Uf = U, Vf = V, Wf = 1.0 + (double)Wd;
+    const double pxe = px0 + (double)dpx, pye = py0 + (double)dpy;
+    b[0] = a.x0 + pxe * a.dx;
+    b[a.ldb] = atan(Uf / Wf);
+    b[2 * a.ldb] = a.y0 + pye * a.dy;
+    b[3 * a.ldb] = atan(Vf / Wf);
+    b[4 * a.ldb] = ph;
+    const double L = (double)ncell - Zs;
+    float *gf = f + 2 * kSeg;
+    gf[0] = (float)(Uf - U0);
+    gf[1] = (float)(Vf - V0);
+    gf[2] = Wd - (float)(W0 - 1.0);
+    gf[3] = (float)(pxe - (px0 + (double)a.sx * U0 / W0 * L));
+    gf[4] = (float)(pye - (py0 + (double)a.sy * V0 / W0 * L));
+    gf[5] = (float)ph;
+    gf[6] = (float)U0;
+    gf[7] = (float)V0;
+    gf[8] = (float)(W0 - 1.0);
+    gf[9] = (float)(px0 - floor(px0));
+    gf[10] = (float)(py0 - floor(py0));
+    gf[11] = S.crossx;
+    gf[12] = S.crossy;
+    gf[13] = S.qx;
+    gf[14] = S.qy;
+    gf[15] = (float)L;
+}
+
+// Block integrals: thread per (column, block); G = sum_c (f[c] + f[c+1]) / 2,
+// M = sum_c (c + 1/2 - zmid) (f[c] + f[c+1]) / 2 + (f[b1] - f[b0]) / 12 over
+// the block's cells c in [b0, b1).
+__global__ void block_kernel(const float4 *__restrict__ f, float4 *__restrict__ G, float4 *__restrict__ M,
+                             int64_t plane, int ncell, int k, int nb) {
+    const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int bb = blockIdx.y;
+    if (col >= plane || bb >= nb) return;
+    const int b0 = bb * k, b1 = min(ncell, b0 + k);
+    const float zm = 0.5f * (float)(b0 + b1);
+    float4 lo = f[(int64_t)b0 * plane + col];
+    const float4 first = lo;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f), m = g;
+    for (int c = b0; c < b1; ++c) {
+        const float4 hi = f[(int64_t)(c + 1) * plane + col];
+        const float4 mid = make_float4(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z),
+                                       0.5f * (lo.w + hi.w));
+        const float zc = (float)c + 0.5f - zm;
+        g.x += mid.x;
+        g.y += mid.y;
+        g.z += mid.z;
+        g.w += mid.w;
+        m.x = fmaf(zc, mid.x, m.x);
+        m.y = fmaf(zc, mid.y, m.y);
+        m.z = fmaf(zc, mid.z, m.z);
+        m.w = fmaf(zc, mid.w, m.w);
+        lo = hi;
+    }
+    const float s12 = 1.0f / 12.0f;
+    m.x = fmaf(lo.x - first.x, s12, m.x);
+    m.y = fmaf(lo.y - first.y, s12, m.y);
+    m.z = fmaf(lo.z - first.z, s12, m.z);
+    m.w = fmaf(lo.w - first.w, s12, m.w);
+    G[(int64_t)bb * plane + col] = g;
+    M[(int64_t)bb * plane + col] = m;
+}
+
+// corrected = base + Y[:, :5] * scale; flag when a predicted error bound
+// (|Y[:, 5]| * e_pos for position, |Y[:, 6]| * e_ang for angle) exceeds its
+// threshold, or when the estimate is not finite.  Flagged ray indices are
+// appended to list (count in *nflag).
+__global__ void apply_kernel(const double *base, int64_t ldb, const float *Y, int64_t ldy, int64_t n, float s_pos,
+                             float s_ang, float s_ph, float e_pos, float e_ang, double t_pos, double t_ang,
+                             double *rf, int64_t ldr, double *phase, int32_t *flag, int32_t *list, int32_t *nflag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *y = Y + i * ldy;
+    const double x = base[i] + (double)(y[0] * s_pos);
+    const double th = base[ldb + i] + (double)(y[2] * s_ang);
+    const double yy = base[2 * ldb + i] + (double)(y[1] * s_pos);
+    const double ph = base[3 * ldb + i] + (double)(y[3] * s_ang);
+    const double pz = base[4 * ldb + i] + (double)(y[4] * s_ph);
+    rf[i] = x;
+    rf[ldr + i] = th;
+    rf[2 * ldr + i] = yy;
+    rf[3 * ldr + i] = ph;
+    if (phase) phase[i] = pz;
+    const double ep = fabs((double)(y[5] * e_pos)), ea = fabs((double)(y[6] * e_ang));
+    const bool bad = !(ep <= t_pos && ea <= t_ang) || isnan(x) || isnan(th) || isnan(yy) || isnan(ph);
+    if (flag) flag[i] = bad ? 1 : 0;
+    if (bad && list) list[atomicAdd(nflag, 1)] = (int32_t)i;
+}
+
+}  // namespace
+
+extern "C" int tt_surrogate_apply(const double *base, int64_t ldb, const float *Y, int64_t ldy, int64_t n,
+                                  const float *scales, const double *thresholds, double *rf, int64_t ldr,
+                                  double *phase, int32_t *flag, int32_t *list, int32_t *nflag, void *stream) {
+    TT_REQUIRE(base && Y && scales && thresholds && rf, "tt_surrogate_apply: null pointer");
+    TT_REQUIRE(n >= 0 && ldb >= n && ldr >= n && ldy >= 7, "tt_surrogate_apply: bad sizes");
+    TT_REQUIRE(!list || nflag, "tt_surrogate_apply: list needs nflag");
+    if (n == 0) return TT_OK;
+    hipLaunchKernelGGL(apply_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream), base,
+                       ldb, Y, ldy, n, scales[0], scales[1], scales[2], scales[3], scales[4], thresholds[0],
+                       thresholds[1], rf, ldr, phase, flag, list, nflag);
+    TT_LAUNCH_CHECK("tt_surrogate_apply launch");
+    return TT_OK;
+}
+
+extern "C" int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t k, float *G4, float *M4,
+                               void *stream) {
+    TT_REQUIRE(field4 && grid && G4 && M4, "tt_field_blocks: null pointer");
+    TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "tt_field_blocks: grid needs >= 2 nodes per axis");
+    TT_REQUIRE(k >= 1 && k <= 1024, "tt_field_blocks: block size k must be in [1, 1024]");
+    const int ncell = grid->nz - 1, nb = (ncell + k - 1) / k;
+    TT_REQUIRE(nb <= 65535, "tt_field_blocks: too many blocks");
+    const int64_t plane = (int64_t)grid->nx * grid->ny;
+    dim3 g((unsigned)tt::ceil_div(plane, 256), (unsigned)nb);
+    hipLaunchKernelGGL(block_kernel, g, dim3(256), 0, tt::as_stream(stream), reinterpret_cast<const float4 *>(field4),
+                       reinterpret_cast<float4 *>(G4), reinterpret_cast<float4 *>(M4), plane, ncell, (int)k, nb);
+    TT_LAUNCH_CHECK("tt_field_blocks launch");
+    return TT_OK;
+}
+
+extern "C" int tt_path_features(const float *field4, const float *G4, const float *M4, int32_t k, const tt_grid *grid,
+                                const double *s0, int64_t ld, int64_t n, const int32_t *order, float *feat,
+                                int64_t ldf, double *base, int64_t ldb, void *stream) {
+    TT_REQUIRE(field4 && G4 && M4 && grid && s0 && feat && base, "tt_path_features: null pointer");
+    TT_REQUIRE(n >= 0 && ld >= n && ldb >= n && ldf >= 48, "tt_path_features: bad sizes");
+    TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "tt_path_features: grid needs >= 2 nodes per axis");
+    TT_REQUIRE(grid->dx > 0 && grid->dy > 0 && grid->dz > 0, "tt_path_features: spacing must be > 0");
+    TT_REQUIRE(k >= 1 && k <= 1024, "tt_path_features: block size k must be in [1, 1024]");
+    if (n == 0) return TT_OK;
+    PathArgs a;
+    a.field = reinterpret_cast<const float4 *>(field4);
+    a.G = reinterpret_cast<const float4 *>(G4);
+    a.M = reinterpret_cast<const float4 *>(M4);
+    a.k = k;
+    a.nb = (grid->nz - 1 + k - 1) / k;
+    a.nx = grid->nx;
+    a.ny = grid->ny;
+    a.nz = grid->nz;
+    a.x0 = grid->x0;
+    a.y0 = grid->y0;
+    a.z0 = grid->z0;
+    a.dx = grid->dx;
+    a.dy = grid->dy;
+    a.dz = grid->dz;
+    a.sx = (float)(grid->dz / grid->dx);
+    a.sy = (float)(grid->dz / grid->dy);
+    a.s0 = s0;
+    a.ld = ld;
+    a.n = n;
+    a.order = order;
+    a.feat = feat;
+    a.ldf = ldf;
+    a.base = base;
+    a.ldb = ldb;
+    hipLaunchKernelGGL(path_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream), a);
+    TT_LAUNCH_CHECK("tt_path_features launch");
+    return TT_OK;
+}

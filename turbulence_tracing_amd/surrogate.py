@@ -1,9 +1,10 @@
-"""Learned correction of the straight-line ray estimate, with the full RK4
-tracer as fallback and judge.
+"""Learned correction of a cheap path-integrated ray estimate, with the full
+RK4 tracer as fallback and judge.
 
 Not part of the reference (which has no surrogate).  The pieces:
-  * ``tt_straight_features`` follows each ray along its unperturbed line and
-    returns a second-order (Born) estimate of the exit state
+  * ``tt_path_features`` integrates the deflection and phase along each ray's
+    path with one field evaluation per z-cell (a drift-kick-drift step; the
+    full tracer takes eight) and returns the estimate of the exit state
     [x, theta, y, phi, phase] plus 48 features;
   * a 4-layer MLP (48 -> 256 -> 256 -> 256 -> 8, GELU, bf16 weights and
     activations, fp32 accumulation; hand-written MFMA GEMMs in libtt)
@@ -31,6 +32,27 @@ N_REG, N_ERR = 5, 2
 
 def _p(t):
     return None if t is None else t.data_ptr()
+
+
+_FRAG_CACHE = {}
+
+
+def _frag_index(layer, fout):
+    """Row / column indices of the packed fragments of one layer: fragment
+    (nt, c) = 64 lanes x 8, lane l element j = W[16 nt + l % 16][k] with
+    k = 32c + 8(l/16) + j for the input layer and k = 32c + 16(j/4) +
+    4(l/16) + j%4 after a hidden layer (the order in which the previous
+    layer's MFMA output sits in the lanes)."""
+    key = (layer, fout)
+    if key not in _FRAG_CACHE:
+        nt_n = (fout + 15) // 16
+        kc = 2 if layer == 0 else 8
+        nt, c, lane, j = np.meshgrid(np.arange(nt_n), np.arange(kc), np.arange(64), np.arange(8), indexing="ij")
+        n = 16 * nt + lane % 16
+        g = lane // 16
+        k = 32 * c + 8 * g + j if layer == 0 else 32 * c + 16 * (j // 4) + 4 * g + j % 4
+        _FRAG_CACHE[key] = (torch.as_tensor(n.reshape(-1)), torch.as_tensor(k.reshape(-1)), nt_n, kc)
+    return _FRAG_CACHE[key]
 
 
 class MLP:
@@ -124,6 +146,38 @@ class MLP:
             self.lib.tt_adam(_p(self.b[l]), _p(self.db[l]), _p(self.mb[l]), _p(self.vb[l]), None, self.b[l].numel(),
                              lr, b1, b2, eps, self.step, 0.0, st)
 
+    def packed(self):
+        """Weights and biases in the fragment layout of tt_mlp_infer (bf16
+        [296 KiB], fp32 [784]); rebuilt after every Adam step."""
+        if getattr(self, "_packed_step", None) == self.step and getattr(self, "_packed", None) is not None:
+            return self._packed
+        if len(self.dims) != 5 or self.dims[1:4] != (256, 256, 256) or self.dims[0] > 64 or self.dims[4] > 16:
+            raise ValueError(f"tt_mlp_infer serves (<=64, 256, 256, 256, <=16) networks, not {self.dims}")
+        ws, bs = [], []
+        for l in range(self.nlayers):
+            n_idx, k_idx, nt, kc = _frag_index(l, self.dims[l + 1])
+            Wp = torch.zeros((nt * 16, kc * 32), dtype=torch.bfloat16, device=self.device)
+            Wp[:self.dims[l + 1], :self.dims[l]] = self.Wb[l]
+            ws.append(Wp[n_idx.to(self.device), k_idx.to(self.device)].reshape(-1))
+            bp = torch.zeros(nt * 16, dtype=torch.float32, device=self.device)
+            bp[:self.dims[l + 1]] = self.b[l]
+            bs.append(bp)
+        self._packed = (torch.cat(ws).contiguous(), torch.cat(bs).contiguous())
+        self._packed_step = self.step
+        return self._packed
+
+    def infer(self, feat, mu, isd, out=None):
+        """Fused forward pass on fp32 features [n, dims[0]] (standardised with
+        mu / isd inside the kernel): fp32 [n, dims[-1]]."""
+        n = feat.shape[0]
+        if feat.stride(1) != 1:
+            raise ValueError("infer: features must be row-contiguous")
+        w, b = self.packed()
+        Y = out if out is not None else torch.empty((n, self.dims[-1]), dtype=torch.float32, device=self.device)
+        self.lib.tt_mlp_infer(_p(feat), feat.stride(0), n, self.dims[0], _p(mu), _p(isd), _p(w), _p(b), _p(Y),
+                              Y.stride(0), self.dims[-1], _lib.stream_ptr())
+        return Y
+
     def state(self):
         return {f"W{l}": self.W[l].cpu().numpy() for l in range(self.nlayers)} | \
                {f"b{l}": self.b[l].cpu().numpy() for l in range(self.nlayers)}
@@ -161,16 +215,34 @@ class Dataset:
     info: dict = field(default_factory=dict)
 
 
-def straight(field4, grid, s0, n=None):
-    """Straight-line estimate base [5, n] (fp64) and features [n, 48] (fp32)."""
+def block_fields(field4, grid, k=16):
+    """Straight-line z-block integrals (G) and first moments (M) of the node
+    field over blocks of k cells, each [ceil((nz - 1) / k), ny, nx, 4]."""
+    lib = _lib.load()
+    nb = -(-(grid.nz - 1) // k)
+    shape = (nb, grid.ny, grid.nx, 4)
+    G = torch.empty(shape, dtype=torch.float32, device=field4.device)
+    M = torch.empty(shape, dtype=torch.float32, device=field4.device)
+    lib.tt_field_blocks(_p(field4), C.byref(grid), k, _p(G), _p(M), _lib.stream_ptr())
+    return G, M
+
+
+def path_estimate(field4, grid, s0, n=None, k=16, blocks=None, order=None):
+    """Cheap path estimate base [5, n] (fp64) and features [n, 48] (fp32)
+    over z-blocks of k cells.  ``blocks`` = block_fields(field4, grid, k)
+    (built here when not given); ``order`` (int32, optional) is the
+    processing order of the rays."""
     lib = _lib.load()
     n = s0.shape[1] if n is None else n
     ld = s0.stride(0)
     dev = s0.device
+    G, M = blocks if blocks is not None else block_fields(field4, grid, k)
+    if G.shape[0] != -(-(grid.nz - 1) // k):
+        raise ValueError(f"path_estimate: blocks were built for another block size than k={k}")
     base = torch.empty((5, n), dtype=torch.float64, device=dev)
     feat = torch.empty((n, NFEAT), dtype=torch.float32, device=dev)
-    lib.tt_straight_features(_p(field4), C.byref(grid), _p(s0), ld, n, _p(feat), NFEAT, _p(base), n,
-                             _lib.stream_ptr())
+    lib.tt_path_features(_p(field4), _p(G), _p(M), k, C.byref(grid), _p(s0), ld, n, _p(order), _p(feat), NFEAT,
+                         _p(base), n, _lib.stream_ptr())
     return base, feat
 
 
@@ -179,9 +251,9 @@ def residual(rf, phase, base):
     return torch.stack([rf[0] - base[0], rf[2] - base[2], rf[1] - base[1], rf[3] - base[3], phase - base[4]])
 
 
-def build_dataset(n_fields, rays_per_field, cfg=None, seed=1000, device="cuda", log=None):
+def build_dataset(n_fields, rays_per_field, cfg=None, seed=1000, device="cuda", log=None, k=16):
     """Rays traced by the tracer through ``n_fields`` seeded fields: features
-    of their straight-line estimate and the residual of the full trace."""
+    of their cheap path estimate and the residual of the full trace."""
     cfg = cfg or Config()
     cfg = Config(**{**cfg.__dict__, "n_rays": rays_per_field})
     p = Pipeline(cfg, device)
@@ -193,7 +265,7 @@ def build_dataset(n_fields, rays_per_field, cfg=None, seed=1000, device="cuda", 
         p.beam(s + 1)
         p.sort()
         p.trace()
-        base, feat = straight(p.field4, p.grid, p.s0)
+        base, feat = path_estimate(p.field4, p.grid, p.s0, k=k)
         feats.append(feat)
         resids.append(residual(p.rf, p.phase, base))
         if log and (f + 1) % max(1, n_fields // 8) == 0:
@@ -225,8 +297,9 @@ def normalise_targets(resid, norm):
 
 
 class Surrogate:
-    def __init__(self, norm, mlp=None, device="cuda", seed=0):
+    def __init__(self, norm, mlp=None, device="cuda", seed=0, k=16):
         self.norm = norm
+        self.k = k  # z-block size of the path estimate the network corrects
         self.device = device
         self.mlp = mlp or MLP(device=device, seed=seed)
         self.lib = _lib.load()
@@ -275,14 +348,9 @@ class Surrogate:
                 log(f"epoch {ep + 1}/{epochs}: mse {l[0]:.4g}  err-head L1 {l[1]:.4g}")
         return hist
 
-    def predict(self, feat, chunk=1 << 21):
-        n = feat.shape[0]
-        Y = torch.empty((n, NOUT), dtype=torch.float32, device=self.device)
-        for i in range(0, n, chunk):
-            j = min(n, i + chunk)
-            X = self.standardize(feat[i:j])
-            self.mlp.forward(X, out=Y[i:j])
-        return Y
+    def predict(self, feat, out=None):
+        """Network output [n, 8] for fp32 features [n, 48] (fused kernel)."""
+        return self.mlp.infer(feat, self.mu, self.isd, out=out)
 
     def apply(self, base, Y, t_pos=1e-6, t_ang=1e-6, want_list=True):
         n = base.shape[1]
@@ -297,45 +365,54 @@ class Surrogate:
                                     _p(cnt), _lib.stream_ptr())
         return rf, phase, flag, lst, cnt
 
-    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, timings=None):
-        """Surrogate + fallback trace of rays s0 (6 x n, device): returns rf
-        (4 x n), phase (n) and the flag per ray (1 = re-traced)."""
+    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, blocks=None, timings=None):
+        """Surrogate + fallback trace of rays s0 (6 x n, device): the block
+        path estimate, the network's correction, and the full RK4 tracer on
+        the rays the network flags (predicted position error above t_pos
+        metres or angle error above t_ang radians, or a non-finite
+        estimate).  Returns rf (4 x n), phase (n) and the flag per ray
+        (1 = re-traced).  ``timings`` (a dict) receives per-stage ms."""
         n = s0.shape[1]
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)] if timings is not None else None
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)] if timings is not None else None
         if ev:
             ev[0].record()
-        base, feat = straight(field4, grid, s0)
+        if blocks is None:
+            blocks = block_fields(field4, grid, self.k)
         if ev:
             ev[1].record()
-        Y = self.predict(feat)
+        base, feat = path_estimate(field4, grid, s0, k=self.k, blocks=blocks)
         if ev:
             ev[2].record()
+        Y = self.predict(feat)
+        if ev:
+            ev[3].record()
         rf, phase, flag, lst, cnt = self.apply(base, Y, t_pos, t_ang)
         nf = int(cnt.item())
         if ev:
-            ev[3].record()
+            ev[4].record()
         if nf > 0:
             opts = _lib.TraceOpts(h, 0.0, 0)
             self.lib.tt_trace_rk4(_p(field4), C.byref(grid), C.byref(opts), _p(s0), s0.stride(0), nf, _p(lst),
                                   _p(rf), n, _p(phase), None, n, None, _lib.stream_ptr())
         if ev:
-            ev[4].record()
+            ev[5].record()
             torch.cuda.synchronize()
-            timings.update(straight=ev[0].elapsed_time(ev[1]), mlp=ev[1].elapsed_time(ev[2]),
-                           apply=ev[2].elapsed_time(ev[3]), fallback=ev[3].elapsed_time(ev[4]), n_fallback=nf)
+            timings.update(blocks=ev[0].elapsed_time(ev[1]), path=ev[1].elapsed_time(ev[2]),
+                           mlp=ev[2].elapsed_time(ev[3]), apply=ev[3].elapsed_time(ev[4]),
+                           fallback=ev[4].elapsed_time(ev[5]), n_fallback=nf)
         return rf, phase, flag
 
     def save(self, path):
         np.savez(path, **self.mlp.state(), mu=self.norm.mu, isd=self.norm.isd,
                  scales=np.array([self.norm.s_pos, self.norm.s_ang, self.norm.s_ph, self.norm.e_pos,
-                                  self.norm.e_ang]))
+                                  self.norm.e_ang]), k=np.array(self.k))
 
     @classmethod
     def load(cls, path, device="cuda"):
         st = np.load(path)
         sc = st["scales"]
         norm = Norm(st["mu"], st["isd"], float(sc[0]), float(sc[1]), float(sc[2]), float(sc[3]), float(sc[4]))
-        s = cls(norm, device=device)
+        s = cls(norm, device=device, k=int(st["k"]))
         s.mlp.load_state(st)
         return s
 
