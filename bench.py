@@ -15,6 +15,10 @@ duration from HIP events on the launch stream, against 8 TB/s.
 cpu_baseline = the reference's CPU path (as-shipped solve_ivp RK45 through
 RegularGridInterpolator, multiprocessing pool) timed on this host on a sample
 of the same rays through the same field.
+surrogate = the learned-correction form on the same realisation: z-block path
+estimate + MLP + RK4 fallback on flagged rays, its time against the plain
+trace (the speedup), its error against the plain trace of the same rays, and
+what the network gains over the path estimate alone (error RMS ratios).
 """
 import argparse
 import json
@@ -40,6 +44,9 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU rounds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="tracer kernel: 0 cell-cached, 1 per-stage gathers")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate + fallback leg")
+    ap.add_argument("--sur-fields", type=int, default=48, help="training realisations for the surrogate")
+    ap.add_argument("--sur-block", type=int, default=16, help="z-block size (cells) of the path estimate")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -154,12 +161,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(cfg),
+            "traffic_source": "profiles/trace_rk4_512_1e7_hbm_traffic.md (FETCH_SIZE x 2 + WRITE_SIZE x 1, "
+                              "calibrated on a 1 GiB copy)",
             "trace_ms": trace_ms,
             "bytes_per_ray_step": BYTES_PER_RAY_STEP,
         },
         "cpu_baseline": None,
     }
+
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_leg(p, cfg, args.sur_fields, args.sur_block, stages)
 
     if base is not None:
         log(f"cpu baseline: {base.P} processes, bundles of {args.cpu_bundle} rays, ~{args.cpu_budget:.0f} s")
@@ -168,6 +180,92 @@ def main():
         finally:
             base.close()
     print(json.dumps(out), flush=True)
+
+
+def profiled_traffic(cfg):
+    """HBM bytes per trace launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a plain copy) for the 512^3 / 1e7-ray workload,
+    read from its committed summary; None for other workloads."""
+    if (cfg.n, cfg.n_rays, cfg.variant) != (512, 10_000_000, 0):
+        return None
+    path = os.path.join(ROOT, "profiles", "trace_rk4_512_1e7_hbm_traffic.md")
+    try:
+        for line in open(path):
+            if line.startswith("trace_rk4_traffic_bytes:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def surrogate_leg(p, cfg, n_fields, k, stages):
+    """Surrogate + fallback against the plain tracer on the bench workload
+    (the realisation of the stage pass): the network is trained on rays the
+    tracer traces through n_fields other seeded realisations; the speedup is
+    plain trace time / surrogate time (block fields, path estimate, MLP,
+    apply, fallback) at the error the surrogate reaches against the plain
+    trace of the same rays."""
+    import numpy as np
+    import torch
+    from turbulence_tracing_amd.pipeline import Config
+    from turbulence_tracing_amd.surrogate import (Surrogate, build_dataset, errors, make_norm, normalise_targets,
+                                                  path_estimate, block_fields, summarize)
+    t0 = time.perf_counter()
+    tcfg = Config(**{**cfg.__dict__, "pix": 64})
+    feat, resid, _ = build_dataset(n_fields, 100_000, tcfg, seed=50_000, k=k)
+    norm = make_norm(feat, resid)
+    sur = Surrogate(norm, seed=1, k=k)
+    hist = sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=5e-4)
+    del feat, resid
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    log(f"surrogate: trained on {n_fields} x 1e5 rays in {train_s:.1f} s")
+    stream = torch.cuda.current_stream()
+    thr = (1e-6, 1e-6)
+
+    def timed(fn, reps=3):
+        ms = []
+        for _ in range(reps):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            r = fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms.append(a.elapsed_time(b))
+        return float(np.mean(ms[1:] if reps > 1 else ms)), r
+
+    plain_ms, _ = timed(p.trace)
+    tim = {}
+    sur_ms, (rf, ph, flag) = timed(lambda: sur.trace(p.field4, p.grid, p.s0, t_pos=thr[0], t_ang=thr[1]))
+    sur.trace(p.field4, p.grid, p.s0, t_pos=thr[0], t_ang=thr[1], timings=tim)
+    e = errors(rf, ph, p.rf, p.phase)
+    # what the network gains over the estimate alone, on this held-out realisation
+    base, f2 = path_estimate(p.field4, p.grid, p.s0, k=k)
+    Y = sur.predict(f2)
+    rfn, phn, _, _, _ = sur.apply(base, Y, 1.0, 1.0)
+    eb = errors(torch.stack([base[0], base[1], base[2], base[3]]), base[4], p.rf, p.phase)
+    en = errors(rfn, phn, p.rf, p.phase)
+    rms = lambda v: float(v.double().pow(2).mean().sqrt())
+    um = lambda d: {a: round(b * 1e6, 5) for a, b in d.items()}
+    res = {
+        "speedup": plain_ms / sur_ms,
+        "plain_trace_ms": plain_ms,
+        "surrogate_ms": sur_ms,
+        "stage_ms": {a: round(b, 4) for a, b in tim.items() if a != "n_fallback"},
+        "rays_fallback": int(tim.get("n_fallback", 0)),
+        "thresholds": {"pos_m": thr[0], "ang_rad": thr[1]},
+        "err_vs_plain": {"pos_um": um(summarize(e[0])), "ang_urad": um(summarize(e[1])),
+                         "phase_mrad": {a: round(b * 1e3, 5) for a, b in summarize(e[2]).items()}},
+        "step_ms_with_surrogate": sum(v for a, v in stages.items() if a != "trace") + sur_ms,
+        "network_gain_rms": {"pos": rms(en[0]) / rms(eb[0]), "ang": rms(en[1]) / rms(eb[1]),
+                             "phase": rms(en[2]) / rms(eb[2])},
+        "block_cells": k,
+        "training": {"fields": n_fields, "rays_per_field": 100_000, "epochs": 6, "seconds": round(train_s, 2),
+                     "final_loss": [float(v) for v in hist[-1]]},
+    }
+    log(f"surrogate: {sur_ms:.2f} ms vs plain {plain_ms:.2f} ms (x{res['speedup']:.2f}); "
+        f"max err pos {res['err_vs_plain']['pos_um'].get('max')} um, ang {res['err_vs_plain']['ang_urad'].get('max')} urad")
+    return res
 
 
 def cpu_baseline(base, p, cfg, bundle, budget):

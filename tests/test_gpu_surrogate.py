@@ -177,29 +177,41 @@ def test_path_features_match_oracle(lib, k, inside):
 
 
 def test_surrogate_gain_and_fallback(lib):
-    """Trained on rays the tracer generates, the network reduces the error of
-    the straight-line estimate on held-out fields; with a zero threshold every
-    ray falls back and the result is the full trace."""
+    """Trained on rays the tracer generates through 32 seeded realisations,
+    the network corrects the block path estimate on a held-out realisation:
+    it removes most of the phase residual and does not degrade positions or
+    angles.  Flagged rays are re-traced by the full tracer, bit for bit; with
+    zero thresholds every ray falls back and the result is the tracer's."""
     from turbulence_tracing_amd.pipeline import Config, Pipeline
     from turbulence_tracing_amd.surrogate import (Surrogate, build_dataset, errors, make_norm, normalise_targets,
-                                                  residual, path_estimate)
-    cfg = Config(n=64, pix=64)
-    feat, resid, _ = build_dataset(12, 40000, cfg, seed=100)
+                                                  path_estimate)
+    cfg = Config(n=128, pix=64)
+    feat, resid, _ = build_dataset(32, 50000, cfg, seed=100, k=8)
     norm = make_norm(feat, resid)
-    sur = Surrogate(norm, seed=1)
-    sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=2e-3)
-    # held-out field
-    p = Pipeline(Config(n=64, pix=64, n_rays=50000))
+    sur = Surrogate(norm, seed=1, k=8)
+    sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=5e-4)
+    p = Pipeline(Config(n=128, pix=64, n_rays=200000))
     p.run(seed=999)
-    base, f2 = path_estimate(p.field4, p.grid, p.s0)
+    base, f2 = path_estimate(p.field4, p.grid, p.s0, k=8)
     Y = sur.predict(f2)
     rf, ph, flag, lst, cnt = sur.apply(base, Y, t_pos=1.0, t_ang=1.0)
     e_base = errors(torch.stack([base[0], base[1], base[2], base[3]]), base[4], p.rf, p.phase)
     e_mlp = errors(rf, ph, p.rf, p.phase)
     rms = lambda e: float(e.double().pow(2).mean().sqrt())
-    assert rms(e_mlp[1]) < 0.7 * rms(e_base[1])      # angle
-    assert rms(e_mlp[0]) < 0.7 * rms(e_base[0])      # position
+    gain = [rms(e_mlp[i]) / rms(e_base[i]) for i in range(3)]
+    print(f"network / estimate error RMS: position {gain[0]:.3f} angle {gain[1]:.3f} phase {gain[2]:.3f}")
+    assert gain[2] < 0.5
+    assert gain[0] < 1.05 and gain[1] < 1.05
+    # an angle threshold at the 90th percentile of the network's own error
+    # bound flags about a tenth of the rays; those are exactly the tracer's
+    t_ang = float(torch.quantile((Y[:100000, 6].abs() * norm.e_ang).double(), 0.9))
+    rf3, ph3, flag3 = sur.trace(p.field4, p.grid, p.s0, t_pos=1.0, t_ang=t_ang)
+    fl = flag3.bool()
+    assert 0 < int(fl.sum()) < p.s0.shape[1]
+    np.testing.assert_array_equal(rf3[:, fl].cpu().numpy(), p.rf[:, fl].cpu().numpy())
+    np.testing.assert_array_equal(ph3[fl].cpu().numpy(), p.phase[fl].cpu().numpy())
     # fallback forced from the host: every ray re-traced -> exactly the tracer
     rf2, ph2, flag2 = sur.trace(p.field4, p.grid, p.s0, t_pos=0.0, t_ang=0.0)
     assert int(flag2.sum()) == p.s0.shape[1]
     np.testing.assert_array_equal(rf2.cpu().numpy(), p.rf.cpu().numpy())
+    np.testing.assert_array_equal(ph2.cpu().numpy(), p.phase.cpu().numpy())
