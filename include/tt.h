@@ -119,6 +119,44 @@ int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *
                  double *rf, int64_t ld_rf, double *phase, double *sf, int64_t ld_sf,
                  int32_t *status, void *stream);
 
+/* Ensembles: the rays of many realisations (uneven grids and ray counts)
+ * traced in ONE persistent launch.  No reference counterpart as a call: it
+ * replaces running ElectronCube.solve once per realisation, as the reference's
+ * drivers do in a loop or across MPI ranks (particle_tracking/
+ * multiprocessing_example.py:41-62, mpi_example.py:82-149).  Each item is one
+ * tt_trace_rk4 call's arguments (no sf / status).  tt_batch_prepare uploads the
+ * items and a chunk table (chunk rays per entry, items with the most z-steps
+ * first) into `workspace` with synchronous copies and fills `info`;
+ * tt_batch_trace is stream-ordered and allocation-free, so it can be captured
+ * in a graph: it resets the chunk counter with a kernel and launches
+ * blocks_per_cu (0 = the occupancy limit) x CUs blocks that take chunks until
+ * none is left.  Results equal tt_trace_rk4's for the same variant. */
+typedef struct {
+    const float *field4;
+    tt_grid grid;
+    const double *s0;
+    int64_t ld_s0, n;
+    const int32_t *order;
+    double *rf;
+    int64_t ld_rf;
+    double *phase;
+} tt_batch_item;
+
+typedef struct {
+    int32_t nitems, nchunks, chunk;
+} tt_batch_info;
+
+typedef struct {
+    int32_t block;         /* threads per block: 64, 128 or 256 */
+    int32_t blocks_per_cu; /* persistent blocks per CU, 0 = occupancy limit */
+    int32_t variant;       /* 0 cell-cached, 1 per-stage gathers (as tt_trace_opts) */
+} tt_batch_tuning;
+
+int tt_batch_workspace_bytes(const tt_batch_item *items, int32_t nitems, int32_t chunk, size_t *bytes);
+int tt_batch_prepare(const tt_batch_item *items, int32_t nitems, double h, int32_t chunk, void *workspace,
+                     size_t bytes, tt_batch_info *info);
+int tt_batch_trace(void *workspace, const tt_batch_info *info, const tt_batch_tuning *tune, void *stream);
+
 /* Trilinear samples of the node field at n points pts (3 x ld float64, m):
  * out (4 x ld_out float64) = the four field channels, zero strictly outside
  * the grid and NaN for NaN points -- the RegularGridInterpolator semantics of

@@ -63,6 +63,20 @@ class GemmArgs(C.Structure):
                 ("ldc", C.c_int64), ("Zpre", C.c_void_p), ("ksplit", C.c_int32)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("field4", C.c_void_p), ("grid", Grid), ("s0", C.c_void_p), ("ld_s0", C.c_int64),
+                ("n", C.c_int64), ("order", C.c_void_p), ("rf", C.c_void_p), ("ld_rf", C.c_int64),
+                ("phase", C.c_void_p)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("nitems", C.c_int32), ("nchunks", C.c_int32), ("chunk", C.c_int32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("variant", C.c_int32)]
+
+
 P = C.c_void_p
 I32, I64, U64, D = C.c_int32, C.c_int64, C.c_uint64, C.c_double
 SIZE_P = C.POINTER(C.c_size_t)
@@ -90,6 +104,9 @@ _SIGNATURES = {
     "tt_bias_grad": [P, I64, I32, I64, P, P],
     "tt_surrogate_loss": [P, I64, P, I64, P, I64, I64, I32, P, I32, P, P, I32, P, P],
     "tt_adam": [P, P, P, P, P, I64, D, D, D, D, I64, D, P],
+    "tt_batch_workspace_bytes": [C.POINTER(BatchItem), I32, I32, C.POINTER(C.c_size_t)],
+    "tt_batch_prepare": [C.POINTER(BatchItem), I32, D, I32, P, C.c_size_t, C.POINTER(BatchInfo)],
+    "tt_batch_trace": [P, C.POINTER(BatchInfo), C.POINTER(BatchTuning), P],
     "tt_mlp_infer": [P, I64, I64, I32, P, P, P, P, P, I64, I32, P],
     "tt_standardize_bf16": [P, I64, I64, I32, P, P, P, I64, P],
 }

@@ -21,6 +21,9 @@
 //
 // Algorithmic traffic (roofline accounting): 4 stages x 8 corners x 16 B =
 // 512 B per ray-step.
+#include <algorithm>
+#include <vector>
+
 #include "tt_internal.h"
 
 namespace {
@@ -251,10 +254,9 @@ __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, floa
     return good;
 }
 
+// One ray: the t-th of the launch (a.order[t] when given).
 template <bool CACHED, bool TAU>
-__global__ __launch_bounds__(256) void trace_rk4_kernel(TraceArgs a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
+__device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
     const int64_t r = a.order ? (int64_t)a.order[t] : t;
     const double *s = a.s0 + r;
     const double x = s[0], y = s[a.ld_s0], z = s[2 * a.ld_s0];
@@ -333,6 +335,44 @@ __global__ __launch_bounds__(256) void trace_rk4_kernel(TraceArgs a) {
         sf[5 * a.ld_sf] = ok ? tt::kC * W : nanv;
     }
     if (a.status) a.status[r] = st;
+}
+
+template <bool CACHED, bool TAU>
+__global__ __launch_bounds__(256) void trace_rk4_kernel(TraceArgs a) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    trace_ray<CACHED, TAU>(a, t);
+}
+
+// ------------------------------------------------------------- ensembles
+// Many realisations in one persistent launch.  The rays of every item are
+// cut into chunks (ordered longest rays first); each block takes the next
+// chunk from a device counter until none is left, so uneven grids and ray
+// counts balance over the whole chip without a launch per item.
+struct BatchHeader {
+    unsigned long long counter;  // next chunk; reset by reset_kernel before each launch
+};
+
+__global__ void reset_kernel(BatchHeader *h) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) h->counter = 0ull;
+}
+
+template <bool CACHED>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_batch_kernel(BatchHeader *hdr, const TraceArgs *__restrict__ items,
+                                                          const int32_t *__restrict__ chunk_item,
+                                                          const int64_t *__restrict__ chunk_start, int nchunks,
+                                                          int chunk) {
+    __shared__ int next;
+    for (;;) {
+        __syncthreads();  // every lane has read `next` of the previous round
+        if (threadIdx.x == 0) next = (int)atomicAdd(&hdr->counter, 1ull);
+        __syncthreads();
+        const int c = __builtin_amdgcn_readfirstlane(next);
+        if (c >= nchunks) return;  // block-uniform: every wave of the block leaves
+        const TraceArgs &a = items[__builtin_amdgcn_readfirstlane(chunk_item[c])];
+        const int64_t s = chunk_start[c], e = min(s + (int64_t)chunk, a.n);
+        for (int64_t t = s + threadIdx.x; t < e; t += blockDim.x) trace_ray<CACHED, false>(a, t);
+    }
 }
 
 // trilinear samples of the node field at arbitrary points (metres)
@@ -433,5 +473,134 @@ extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_t
     else
         hipLaunchKernelGGL((trace_rk4_kernel<false, true>), nblk, dim3(block), 0, st, a);
     TT_LAUNCH_CHECK("tt_trace_rk4 launch");
+    return TT_OK;
+}
+
+namespace {
+constexpr size_t kBatchAlign = 256;
+size_t align_up(size_t v) { return (v + kBatchAlign - 1) / kBatchAlign * kBatchAlign; }
+
+struct BatchLayout {
+    size_t items, citem, cstart, total;
+};
+
+BatchLayout batch_layout(int32_t nitems, int64_t nchunks) {
+    BatchLayout L;
+    L.items = align_up(sizeof(BatchHeader));
+    L.citem = L.items + align_up(sizeof(TraceArgs) * (size_t)nitems);
+    L.cstart = L.citem + align_up(sizeof(int32_t) * (size_t)nchunks);
+    L.total = L.cstart + align_up(sizeof(int64_t) * (size_t)nchunks);
+    return L;
+}
+
+int64_t count_chunks(const tt_batch_item *items, int32_t nitems, int32_t chunk) {
+    int64_t c = 0;
+    for (int i = 0; i < nitems; ++i) c += tt::ceil_div(items[i].n, (int64_t)chunk);
+    return c;
+}
+}  // namespace
+
+extern "C" int tt_batch_workspace_bytes(const tt_batch_item *items, int32_t nitems, int32_t chunk, size_t *bytes) {
+    TT_REQUIRE(items && bytes && nitems >= 1 && chunk >= 1, "tt_batch_workspace_bytes: bad arguments");
+    *bytes = batch_layout(nitems, count_chunks(items, nitems, chunk)).total;
+    return TT_OK;
+}
+
+extern "C" int tt_batch_prepare(const tt_batch_item *items, int32_t nitems, double h, int32_t chunk, void *workspace,
+                                size_t bytes, tt_batch_info *info) {
+    TT_REQUIRE(items && workspace && info && nitems >= 1, "tt_batch_prepare: bad arguments");
+    TT_REQUIRE(chunk >= 64 && chunk <= (1 << 24), "tt_batch_prepare: chunk must be in [64, 2^24]");
+    const double mm = 1.0 / h;
+    const int m = (int)llround(mm);
+    TT_REQUIRE(m >= 1 && m <= 64 && fabs(mm - m) < 1e-9, "tt_batch_prepare: h must be 1/m cells with 1 <= m <= 64");
+    const int64_t nchunks = count_chunks(items, nitems, chunk);
+    TT_REQUIRE(nchunks >= 1 && nchunks < (1ll << 31), "tt_batch_prepare: no rays or too many chunks");
+    const BatchLayout L = batch_layout(nitems, nchunks);
+    TT_REQUIRE(bytes >= L.total, "tt_batch_prepare: workspace of %zu bytes < %zu needed", bytes, L.total);
+    std::vector<TraceArgs> args((size_t)nitems);
+    for (int i = 0; i < nitems; ++i) {
+        const tt_batch_item &it = items[i];
+        TT_REQUIRE(it.field4 && it.s0 && it.rf && it.n >= 0, "tt_batch_prepare: item %d: null pointer or n < 0", i);
+        TT_REQUIRE(it.ld_s0 >= it.n && it.ld_rf >= it.n, "tt_batch_prepare: item %d: leading dimension < n", i);
+        TT_REQUIRE((int64_t)(it.grid.nz - 1) * m < (1 << 23), "tt_batch_prepare: item %d: too many steps", i);
+        TraceArgs a{};
+        if (int rc = fill_grid_args(&it.grid, a)) return rc;
+        a.field = reinterpret_cast<const float4 *>(it.field4);
+        a.m = m;
+        a.h = 1.0f / (float)m;
+        a.cT = 0.0;
+        a.s0 = it.s0;
+        a.ld_s0 = it.ld_s0;
+        a.n = it.n;
+        a.order = it.order;
+        a.rf = it.rf;
+        a.ld_rf = it.ld_rf;
+        a.phase = it.phase;
+        a.sf = nullptr;
+        a.ld_sf = 0;
+        a.status = nullptr;
+        args[(size_t)i] = a;
+    }
+    // chunks of the items with the longest rays (most z-steps) first
+    std::vector<int32_t> idx((size_t)nitems);
+    for (int i = 0; i < nitems; ++i) idx[(size_t)i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t p, int32_t q) { return items[p].grid.nz > items[q].grid.nz; });
+    std::vector<int32_t> citem;
+    std::vector<int64_t> cstart;
+    citem.reserve((size_t)nchunks);
+    cstart.reserve((size_t)nchunks);
+    for (int32_t i : idx)
+        for (int64_t s = 0; s < items[i].n; s += chunk) {
+            citem.push_back(i);
+            cstart.push_back(s);
+        }
+    BatchHeader hdr{0ull};
+    char *w = static_cast<char *>(workspace);
+    TT_HIP(hipMemcpy(w, &hdr, sizeof(hdr), hipMemcpyHostToDevice), "tt_batch_prepare: header upload");
+    TT_HIP(hipMemcpy(w + L.items, args.data(), sizeof(TraceArgs) * args.size(), hipMemcpyHostToDevice),
+           "tt_batch_prepare: item upload");
+    TT_HIP(hipMemcpy(w + L.citem, citem.data(), sizeof(int32_t) * citem.size(), hipMemcpyHostToDevice),
+           "tt_batch_prepare: chunk table upload");
+    TT_HIP(hipMemcpy(w + L.cstart, cstart.data(), sizeof(int64_t) * cstart.size(), hipMemcpyHostToDevice),
+           "tt_batch_prepare: chunk table upload");
+    info->nitems = nitems;
+    info->nchunks = (int32_t)nchunks;
+    info->chunk = chunk;
+    return TT_OK;
+}
+
+extern "C" int tt_batch_trace(void *workspace, const tt_batch_info *info, const tt_batch_tuning *tune, void *stream) {
+    TT_REQUIRE(workspace && info && tune, "tt_batch_trace: null pointer");
+    TT_REQUIRE(info->nitems >= 1 && info->nchunks >= 1 && info->chunk >= 64, "tt_batch_trace: info not prepared");
+    TT_REQUIRE(tune->block == 64 || tune->block == 128 || tune->block == 256,
+               "tt_batch_trace: block must be 64, 128 or 256");
+    TT_REQUIRE(tune->variant == 0 || tune->variant == 1, "tt_batch_trace: variant must be 0 or 1");
+    TT_REQUIRE(tune->blocks_per_cu >= 0 && tune->blocks_per_cu <= 64, "tt_batch_trace: blocks_per_cu in [0, 64]");
+    int dev = 0, ncu = 0, occ = 0;
+    TT_HIP(hipGetDevice(&dev), "tt_batch_trace: hipGetDevice");
+    TT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "tt_batch_trace: CU count");
+    if (tune->variant == 0)
+        TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_batch_kernel<true>, tune->block, 0),
+               "tt_batch_trace: occupancy");
+    else
+        TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_batch_kernel<false>, tune->block, 0),
+               "tt_batch_trace: occupancy");
+    const int per_cu = tune->blocks_per_cu > 0 ? std::min(tune->blocks_per_cu, std::max(occ, 1)) : std::max(occ, 1);
+    const dim3 grid((unsigned)(ncu * per_cu));
+    char *w = static_cast<char *>(workspace);
+    BatchHeader *hdr = reinterpret_cast<BatchHeader *>(w);
+    const BatchLayout L = batch_layout(info->nitems, info->nchunks);
+    hipStream_t st = tt::as_stream(stream);
+    hipLaunchKernelGGL(reset_kernel, dim3(1), dim3(64), 0, st, hdr);
+    const TraceArgs *items = reinterpret_cast<const TraceArgs *>(w + L.items);
+    const int32_t *citem = reinterpret_cast<const int32_t *>(w + L.citem);
+    const int64_t *cstart = reinterpret_cast<const int64_t *>(w + L.cstart);
+    if (tune->variant == 0)
+        hipLaunchKernelGGL(trace_batch_kernel<true>, grid, dim3(tune->block), 0, st, hdr, items, citem, cstart,
+                           info->nchunks, info->chunk);
+    else
+        hipLaunchKernelGGL(trace_batch_kernel<false>, grid, dim3(tune->block), 0, st, hdr, items, citem, cstart,
+                           info->nchunks, info->chunk);
+    TT_LAUNCH_CHECK("tt_batch_trace launch");
     return TT_OK;
 }

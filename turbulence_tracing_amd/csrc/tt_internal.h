@@ -29,6 +29,12 @@ inline int hip_check(hipError_t e, const char *what) {
         }                                                                                       \
     } while (0)
 
+#define TT_HIP(call, what)                                                                      \
+    do {                                                                                        \
+        hipError_t _e = (call);                                                                 \
+        if (_e != hipSuccess) return ::tt::hip_check(_e, what);                                 \
+    } while (0)
+
 #define TT_LAUNCH_CHECK(what)                                                                   \
     do {                                                                                        \
         hipError_t _e = hipGetLastError();                                                      \

@@ -43,6 +43,7 @@ class Config:
     fringes: float = 20.0        # interferogram fringes across the detector
     h: float = 0.5               # RK4 step, cells
     variant: int = 0             # tracer kernel: 0 cell-cached, 1 per-stage gathers
+    brick_shift: int = 0         # ray order: entry columns grouped in bricks of 2^shift cells
 
     @property
     def nc(self):
@@ -123,7 +124,8 @@ class Pipeline:
 
     def sort(self):
         N = self.cfg.n_rays
-        self.lib.tt_ray_sort(_lib.ptr(self.s0_raw), N, N, _lib.C.byref(self.grid), 0, _lib.ptr(self.sort_scratch),
+        self.lib.tt_ray_sort(_lib.ptr(self.s0_raw), N, N, _lib.C.byref(self.grid), self.cfg.brick_shift,
+                             _lib.ptr(self.sort_scratch),
                              _lib.ptr(self.order), _lib.stream_ptr())
         torch.index_select(self.s0_raw, 1, self.order.long(), out=self.s0)
 
