@@ -116,31 +116,44 @@ class Pipeline:
                                  2 * np.pi / cfg.lwl, _lib.ptr(self.sumsq), n ** 3, _lib.ptr(self.field4),
                                  _lib.stream_ptr())
 
-    def beam(self, seed):
+    def beam(self, seed, n=None, offset=0):
+        """Rays [offset, offset + n) of the seeded beam (Philox counter =
+        ray index), so waves of one beam are the rays of one large draw."""
         cfg = self.cfg
-        N = cfg.n_rays
-        self.lib.tt_beam_sample(N, cfg.beam_size, cfg.divergence, cfg.L / 2, 2, int(seed), 0, _lib.ptr(self.s0_raw),
-                                N, _lib.stream_ptr())
+        n = cfg.n_rays if n is None else n
+        self.lib.tt_beam_sample(n, cfg.beam_size, cfg.divergence, cfg.L / 2, 2, int(seed), int(offset),
+                                _lib.ptr(self.s0_raw), cfg.n_rays, _lib.stream_ptr())
 
-    def sort(self):
+    def sort(self, n=None):
         N = self.cfg.n_rays
-        self.lib.tt_ray_sort(_lib.ptr(self.s0_raw), N, N, _lib.C.byref(self.grid), self.cfg.brick_shift,
-                             _lib.ptr(self.sort_scratch),
-                             _lib.ptr(self.order), _lib.stream_ptr())
-        torch.index_select(self.s0_raw, 1, self.order.long(), out=self.s0)
+        n = N if n is None else n
+        self.lib.tt_ray_sort(_lib.ptr(self.s0_raw), N, n, _lib.C.byref(self.grid), self.cfg.brick_shift,
+                             _lib.ptr(self.sort_scratch), _lib.ptr(self.order), _lib.stream_ptr())
+        if n == N:
+            torch.index_select(self.s0_raw, 1, self.order.long(), out=self.s0)
+        else:
+            self.s0[:, :n].copy_(torch.index_select(self.s0_raw[:, :n], 1, self.order[:n].long()))
 
-    def trace(self):
+    def trace(self, n=None):
         N = self.cfg.n_rays
+        n = N if n is None else n
         self.lib.tt_trace_rk4(_lib.ptr(self.field4), _lib.C.byref(self.grid), _lib.C.byref(self.opts),
-                              _lib.ptr(self.s0), N, N, None, _lib.ptr(self.rf), N, _lib.ptr(self.phase), None, N,
+                              _lib.ptr(self.s0), N, n, None, _lib.ptr(self.rf), N, _lib.ptr(self.phase), None, N,
                               None, _lib.stream_ptr())
 
-    def detect(self):
-        N = self.cfg.n_rays
+    def zero_histograms(self):
         self.H_shadow.zero_()
         self.H_knife.zero_()
         self.H_interf.zero_()
-        self.lib.tt_detect(self._dets, 3, _lib.ptr(self.rf), N, _lib.ptr(self.phase), N, _lib.stream_ptr())
+
+    def detect(self, n=None, accumulate=False):
+        """Bin rays [0, n) into the three histograms (added to the current
+        counts when accumulate is true)."""
+        N = self.cfg.n_rays
+        n = N if n is None else n
+        if not accumulate:
+            self.zero_histograms()
+        self.lib.tt_detect(self._dets, 3, _lib.ptr(self.rf), N, _lib.ptr(self.phase), n, _lib.stream_ptr())
 
     def run(self, seed=0):
         """One realisation: field seed ``seed``, beam seed ``seed + 1``."""
