@@ -44,6 +44,8 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU rounds")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="tracer kernel: 0 cell-cached, 1 per-stage gathers")
+    ap.add_argument("--workload", choices=["single", "large"], default="single",
+                    help="single: 512^3 / 1e7 rays (default); large: 1024^3 / 1e8 rays streamed in 1e7-ray waves")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate + fallback leg")
     ap.add_argument("--sur-fields", type=int, default=48, help="training realisations for the surrogate")
     ap.add_argument("--sur-block", type=int, default=16, help="z-block size (cells) of the path estimate")
@@ -62,6 +64,9 @@ def main():
         print("bench.py measures one GPU: this path is data-parallel over rays and the machine "
               "has one GPU per job; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "large":
+        return main_large(args)
 
     # the CPU pool forks before this process touches the GPU
     base = None
@@ -180,6 +185,119 @@ def main():
         finally:
             base.close()
     print(json.dumps(out), flush=True)
+
+
+def main_large(args):
+    """1024^3 FP32 field (L = 20 mm, 15% RMS, ne 5e24), 1e8 rays in an 8 mm
+    disc streamed in 1e7-ray waves, 4096^2 histograms.  A step is the whole
+    streamed run (all waves) through the resident field."""
+    import numpy as np
+    import torch
+    from turbulence_tracing_amd.waves import WaveRun, large_volume_config
+    steps = args.steps if args.steps != 5 else 2
+    cfg = large_volume_config(variant=args.variant)
+    run = WaveRun(cfg, 100_000_000, 10_000_000)
+    run.field(7)
+    log(f"large volume ready: {cfg.n}^3, {run.total:.0e} rays in {run.n_waves} waves")
+    stream = torch.cuda.current_stream()
+    for w in range(args.warmup):
+        run.run(8 + w)
+    torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        run.run(100 + k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / steps
+    log(f"timed {steps} streamed runs: {ms_per_step:.1f} ms each")
+    # per-wave trace kernel time (events on the launch stream) for the roofline
+    p = run.p
+    p.beam(100, n=run.wave, offset=0)
+    p.sort()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    p.trace()
+    b.record(stream)
+    torch.cuda.synchronize()
+    trace_ms = a.elapsed_time(b)
+    wave_steps = run.wave * cfg.steps_per_ray
+    achieved = BYTES_PER_RAY_STEP * wave_steps / (trace_ms / 1e3) / 1e9
+    out = {
+        "metric": "ray-steps/s at 1024^3, 1e8 rays in 1e7-ray waves; surrogate vs plain speedup at fixed error",
+        "value": run.ray_steps / (ms_per_step / 1e3),
+        "unit": "ray-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Philox field and beam)",
+        "config": {"workload": f"{cfg.n}^3 FP32 k^-11/3 field (L=20 mm, ne=5e24(1+0.15 d), 1064 nm), 1e8 rays "
+                               f"(8 mm disc, 50 urad) in 10 waves of 1e7, RK4 dz=0.5dx ({cfg.steps_per_ray} "
+                               f"steps/ray), {cfg.pix}^2 histograms",
+                   "grid": cfg.n, "rays": run.total, "wave": run.wave, "steps_per_ray": cfg.steps_per_ray},
+        "rays_per_s": run.total / (ms_per_step / 1e3),
+        "roofline": {"kernel": "trace_rk4_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None, "trace_ms": trace_ms,
+                     "bytes_per_ray_step": BYTES_PER_RAY_STEP},
+        "cpu_baseline": None,
+    }
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_large(run, cfg, ms_per_step)
+    print(json.dumps(out), flush=True)
+
+
+def surrogate_large(run, cfg, plain_ms):
+    """Surrogate + fallback over the streamed run: block size chosen by the
+    error budget on 1e6 probe rays, network trained on 8 other 1024^3
+    realisations, errors measured on the first wave against the plain trace."""
+    import torch
+    from turbulence_tracing_amd.pipeline import Config
+    from turbulence_tracing_amd.surrogate import (Surrogate, block_fields, build_dataset, choose_block, errors,
+                                                  make_norm, normalise_targets, summarize)
+    p = run.p
+    p.beam(100, n=run.wave, offset=0)
+    p.sort()
+    p.trace()
+    rf0, ph0 = p.rf.clone(), p.phase.clone()
+    order0 = p.order.long().clone()
+    probe = 1_000_000
+    k, kstats = choose_block(p.field4, p.grid, p.s0[:, :probe], rf0[:, :probe], ph0[:probe])
+    log(f"surrogate: block size {k} ({kstats})")
+    t0 = time.perf_counter()
+    feat, resid, _ = build_dataset(8, 200_000, Config(**{**cfg.__dict__, "pix": 64}), seed=60_000, k=k)
+    norm = make_norm(feat, resid)
+    sur = Surrogate(norm, seed=1, k=k)
+    sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=5e-4)
+    del feat, resid
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    blocks = block_fields(p.field4, p.grid, k)
+    first = {}
+
+    def grab(w, n):
+        if w == 0:
+            first["rf"], first["ph"], first["order"] = p.rf.clone(), p.phase.clone(), p.order.long().clone()
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nfb = run.run(100, surrogate=sur, blocks=blocks, on_wave=grab)
+    torch.cuda.synchronize()
+    sur_ms = 1e3 * (time.perf_counter() - t0)
+    # both runs hold wave 0 in their own entry-column order: compare ray by ray
+    # in the beam's order
+    def unsort(rf, ph, o):
+        r, q = torch.empty_like(rf), torch.empty_like(ph)
+        r[:, o] = rf
+        q[o] = ph
+        return r, q
+    e = errors(*unsort(first["rf"], first["ph"], first["order"]), *unsort(rf0, ph0, order0))
+    um = lambda d: {a: round(b * 1e6, 5) for a, b in d.items()}
+    res = {"speedup": plain_ms / sur_ms, "plain_ms": plain_ms, "surrogate_ms": sur_ms, "block_cells": k,
+           "block_probe": {str(a): [round(x * 1e6, 4) for x in v] for a, v in kstats.items()},
+           "rays_fallback": nfb, "err_vs_plain_first_wave": {"pos_um": um(summarize(e[0])),
+                                                             "ang_urad": um(summarize(e[1]))},
+           "training": {"fields": 8, "rays_per_field": 200_000, "seconds": round(train_s, 2)}}
+    log(f"surrogate: {sur_ms:.0f} ms vs plain {plain_ms:.0f} ms (x{res['speedup']:.2f})")
+    return res
 
 
 def profiled_traffic(cfg):

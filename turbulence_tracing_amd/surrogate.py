@@ -246,6 +246,27 @@ def path_estimate(field4, grid, s0, n=None, k=16, blocks=None, order=None):
     return base, feat
 
 
+def choose_block(field4, grid, s0, rf_ref, phase_ref, candidates=(32, 16, 8, 4, 2, 1), p999_max=0.5e-6,
+                 max_ang=1e-6, max_pos=1e-6):
+    """Largest z-block size whose path estimate, on probe rays s0 already
+    traced by the full tracer (rf_ref, phase_ref), keeps the angle error's
+    99.9th percentile within p999_max and its maximum within max_ang
+    (radians), and the position error within max_pos (metres).  Falls back
+    to k = 1.  Returns (k, {k: (p999, max angle, max position)})."""
+    stats = {}
+    for k in sorted(candidates, reverse=True):
+        base, _ = path_estimate(field4, grid, s0, k=k)
+        ep, ea, _ = errors(torch.stack([base[0], base[1], base[2], base[3]]), base[4], rf_ref, phase_ref)
+        ea = ea[torch.isfinite(ea)].double()
+        ep = ep[torch.isfinite(ep)].double()
+        sub = ea[:1 << 22]
+        p999 = float(torch.quantile(sub, 0.999)) if sub.numel() else 0.0
+        stats[k] = (p999, float(ea.max()) if ea.numel() else 0.0, float(ep.max()) if ep.numel() else 0.0)
+        if stats[k][0] <= p999_max and stats[k][1] <= max_ang and stats[k][2] <= max_pos:
+            return k, stats
+    return 1, stats
+
+
 def residual(rf, phase, base):
     """[5, n] fp64 residual full - estimate, in (x, y, theta, phi, phase) order."""
     return torch.stack([rf[0] - base[0], rf[2] - base[2], rf[1] - base[1], rf[3] - base[3], phase - base[4]])
