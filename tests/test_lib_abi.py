@@ -57,3 +57,38 @@ def test_optics_op_tables_match_header():
         assert re.search(rf"TT_OP_{name}\s*=\s*{val}\b", src)
         assert getattr(_lib, "OP_" + name) == val
     assert re.search(rf"#define TT_MAX_OPS {_lib.TT_MAX_OPS}\b", src)
+
+
+# ctypes mirror of every struct in include/tt.h: sizes and field offsets must
+# equal what the C compiler lays out (a mismatch corrupts every call)
+STRUCTS = {
+    "tt_grid": "Grid", "tt_spectrum": "Spectrum", "tt_trace_opts": "TraceOpts", "tt_batch_item": "BatchItem",
+    "tt_batch_info": "BatchInfo", "tt_batch_tuning": "BatchTuning", "tt_optic": "Optic", "tt_train": "Train",
+    "tt_hist": "Hist", "tt_detector": "Detector", "tt_gemm_args": "GemmArgs",
+}
+
+
+def test_struct_layouts_match_c(tmp_path):
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "tt.h"', 'int main(void) {']
+    for cname, pyname in STRUCTS.items():
+        st = getattr(_lib, pyname)
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in st._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, pyname in STRUCTS.items():
+        st = getattr(_lib, pyname)
+        assert got[(cname, "size")] == _lib.C.sizeof(st), cname
+        for f in st._fields_:
+            assert got[(cname, f[0])] == getattr(st, f[0]).offset, (cname, f[0])
