@@ -239,30 +239,33 @@ int tt_detect(const tt_detector *dets, int32_t ndet, const double *rf, int64_t l
 
 /* Straight-line z-block integrals of the node field for tt_path_features:
  * block b covers cells [b k, min((b + 1) k, nz - 1)); per node column,
- * G4[b] = sum over its cells of the cell's mid-plane value (the mean of the
+ * G = sum over its cells of the cell's mid-plane value (the mean of the
  * cell's two node planes, the exact z-integral of the trilinear field) and
- * M4[b] = the exact first moment about the block centre,
- * sum_c (c + 1/2 - zmid) mid_c + (f[b1] - f[b0]) / 12.  Both
- * ceil((nz - 1) / k) * ny * nx float4. */
-int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t k, float *G4, float *M4, void *stream);
+ * M = the exact first moment about the block centre,
+ * sum_c (c + 1/2 - zmid) mid_c + (f[b1] - f[b0]) / 12.  GM4 is
+ * [ny][nx][nb][2] float4 ({G, M} per column and block, a column's blocks
+ * consecutive), nb = ceil((nz - 1) / k). */
+int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t k, float *GM4, void *stream);
 
 /* Cheap path estimate of each ray's exit state and its 48 surrogate
  * features: one drift-kick-drift step per z-block of k cells (the block
- * integral G4 at the mid-block position, plus v . grad(M4) for the ray's
- * transverse drift in the block; the exit position gets -s M4_x / W for a
- * kick spread over the block), against the eight field evaluations per z-cell
+ * integral G at the mid-block position, plus v . grad(M) for the ray's
+ * transverse drift in the block; the exit position gets -s M_x / W for a
+ * kick spread over the block; G, M from tt_field_blocks), against the eight
+ * field evaluations per z-cell
  * of tt_trace_rk4.  A ray entering inside the grid takes single-cell steps
  * from field4 up to the next block boundary.  order (nullable) as in
- * tt_trace_rk4.  base: 5 x ldb float64 [x, theta, y, phi, phase]; feat: n rows
- * of ldf >= 48 float32: 16 z-segment sums of the x kicks, 16 of the y kicks,
+ * tt_trace_rk4.  base: 5 x ldb float64 [x, theta, y, phi, phase]; feat: 48
+ * float32 features per ray stored feature-major, feature q of ray r at
+ * feat[q * ldf + r] (ldf >= n): 16 z-segment sums of the x kicks, 16 of the y kicks,
  * then U - U0, V - V0, W - W0, the exit displacement from the entry line (x,
  * y, cells), the phase, U0, V0, W0 - 1, the entry-cell fractions x, y, the x
  * and y column changes between steps, the summed second-order proxies
  * (dGx/dx Gx + dGx/dy Gy) / W^2 and (dGy/dx Gx + dGy/dy Gy) / W^2, and the
  * path length in cells. */
-int tt_path_features(const float *field4, const float *G4, const float *M4, int32_t k, const tt_grid *grid,
-                     const double *s0, int64_t ld, int64_t n, const int32_t *order, float *feat, int64_t ldf,
-                     double *base, int64_t ldb, void *stream);
+int tt_path_features(const float *field4, const float *GM4, int32_t k, const tt_grid *grid, const double *s0,
+                     int64_t ld, int64_t n, const int32_t *order, float *feat, int64_t ldf, double *base, int64_t ldb,
+                     void *stream);
 
 /* Apply the MLP output Y (n rows, ld ldy >= 7, fp32) to the cheap path
  * estimate: rf = base[0:4] + Y[:, {0, 2, 1, 3}] * {s_pos, s_ang, s_pos, s_ang}
@@ -270,22 +273,24 @@ int tt_path_features(const float *field4, const float *G4, const float *M4, int3
  * e_pos, e_ang}.  A ray is flagged for the full tracer when
  * |Y[:, 5]| e_pos > thresholds[0] (metres) or |Y[:, 6]| e_ang > thresholds[1]
  * (radians) or the estimate is not finite; flag (optional) gets 0/1 per ray
- * and the flagged indices are appended to list (optional) with *nflag (device
- * int32, caller-zeroed) counting them. */
+ * and first + (index of each flagged ray) is appended to list (optional) with
+ * *nflag (device int32, caller-zeroed) counting them, so batches of one ray
+ * set can share one list. */
 int tt_surrogate_apply(const double *base, int64_t ldb, const float *Y, int64_t ldy, int64_t n, const float *scales,
                        const double *thresholds, double *rf, int64_t ldr, double *phase, int32_t *flag,
-                       int32_t *list, int32_t *nflag, void *stream);
+                       int32_t *list, int32_t *nflag, int64_t first, void *stream);
 
 /* Fused inference of the surrogate network nfeat -> 256 -> 256 -> 256 -> nout
  * (GELU, tanh form, on the hidden layers; bf16 MFMA, fp32 accumulation):
- * Y[r][o] for fp32 features feat[r * ldf + f] standardised as
+ * Y[r][o] for fp32 features feat[r * srow + f * scol] standardised as
  * (x - mu[f]) * inv_sd[f] and rounded to bf16.  nfeat <= 64, nout <= 16.
  * w_packed: bf16 fragments, layer by layer (tiles x chunks 16 x 2, 16 x 8,
  * 16 x 8, 1 x 8), fragment (nt, c) = 64 lanes x 8 with lane l element j =
  * W[16 nt + l%16][k], k = 32c + 8(l/16) + j for the first layer and
  * 32c + 16(j/4) + 4(l/16) + j%4 for the others (zero beyond the layer's
  * size); bias_packed: fp32 256 + 256 + 256 + 16.  Both 16-byte aligned. */
-int tt_mlp_infer(const float *feat, int64_t ldf, int64_t n, int32_t nfeat, const float *mu, const float *inv_sd,
+int tt_mlp_infer(const float *feat, int64_t srow, int64_t scol, int64_t n, int32_t nfeat, const float *mu,
+                 const float *inv_sd,
                  const void *w_packed, const float *bias_packed, float *Y, int64_t ldy, int32_t nout, void *stream);
 
 /* C[m][n] = act(sum_k A(m,k) B(k,n) + bias[n]) on bf16 MFMA with fp32
@@ -326,8 +331,9 @@ int tt_surrogate_loss(const float *Y, int64_t ldy, const float *T, int64_t ldt, 
  * bf16 copy w_bf16 when non-NULL. */
 int tt_adam(float *w, const float *g, float *m, float *v, void *w_bf16, int64_t n, double lr, double b1, double b2,
             double eps, int64_t step, double weight_decay, void *stream);
-/* out (bf16, [rows][ldo], zero-padded) = (x - mu) * inv_sd over cols */
-int tt_standardize_bf16(const float *x, int64_t ldx, int64_t rows, int32_t cols, const float *mu,
+/* out (bf16, [rows][ldo], zero-padded) = (x - mu) * inv_sd over cols, with
+ * x element (r, c) at x[r * srow + c * scol] */
+int tt_standardize_bf16(const float *x, int64_t srow, int64_t scol, int64_t rows, int32_t cols, const float *mu,
                         const float *inv_sd, void *out_bf16, int64_t ldo, void *stream);
 
 #ifdef __cplusplus

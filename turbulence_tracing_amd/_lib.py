@@ -96,9 +96,9 @@ _SIGNATURES = {
     "tt_ray_sort": [P, I64, I64, C.POINTER(Grid), I32, P, P, P],
     "tt_optics_apply": [C.POINTER(Train), P, I64, I64, P, I64, P],
     "tt_detect": [C.POINTER(Detector), I32, P, I64, P, I64, P],
-    "tt_field_blocks": [P, C.POINTER(Grid), I32, P, P, P],
-    "tt_path_features": [P, P, P, I32, C.POINTER(Grid), P, I64, I64, P, P, I64, P, I64, P],
-    "tt_surrogate_apply": [P, I64, P, I64, I64, P, P, P, I64, P, P, P, P, P],
+    "tt_field_blocks": [P, C.POINTER(Grid), I32, P, P],
+    "tt_path_features": [P, P, I32, C.POINTER(Grid), P, I64, I64, P, P, I64, P, I64, P],
+    "tt_surrogate_apply": [P, I64, P, I64, I64, P, P, P, I64, P, P, P, P, I64, P],
     "tt_gemm_bf16": [C.POINTER(GemmArgs), P],
     "tt_gelu_backward": [P, P, P, P, I64, P],
     "tt_bias_grad": [P, I64, I32, I64, P, P],
@@ -107,8 +107,8 @@ _SIGNATURES = {
     "tt_batch_workspace_bytes": [C.POINTER(BatchItem), I32, I32, C.POINTER(C.c_size_t)],
     "tt_batch_prepare": [C.POINTER(BatchItem), I32, D, I32, P, C.c_size_t, C.POINTER(BatchInfo)],
     "tt_batch_trace": [P, C.POINTER(BatchInfo), C.POINTER(BatchTuning), P],
-    "tt_mlp_infer": [P, I64, I64, I32, P, P, P, P, P, I64, I32, P],
-    "tt_standardize_bf16": [P, I64, I64, I32, P, P, P, I64, P],
+    "tt_mlp_infer": [P, I64, I64, I64, I32, P, P, P, P, P, I64, I32, P],
+    "tt_standardize_bf16": [P, I64, I64, I64, I32, P, P, P, I64, P],
 }
 
 # every symbol include/tt.h declares

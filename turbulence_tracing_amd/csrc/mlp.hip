@@ -236,14 +236,14 @@ __global__ void adam_kernel(float *w, const float *g, float *m, float *v, __bf16
 }
 
 // (x - mu) / sigma -> bf16, row-major [rows][cols] into [rows][ldo] (zero pad)
-__global__ void standardize_kernel(const float *x, int64_t ldx, int64_t rows, int cols, const float *mu,
-                                   const float *isd, __bf16 *out, int64_t ldo) {
+__global__ void standardize_kernel(const float *x, int64_t srow, int64_t scol, int64_t rows, int cols,
+                                   const float *mu, const float *isd, __bf16 *out, int64_t ldo) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t r = i / ldo;
     const int c = (int)(i - r * ldo);
     if (r >= rows) return;
     float v = 0.f;
-    if (c < cols) v = (x[r * ldx + c] - mu[c]) * isd[c];
+    if (c < cols) v = (x[r * srow + c * scol] - mu[c]) * isd[c];
     out[i] = (__bf16)v;
 }
 
@@ -269,8 +269,8 @@ constexpr int kInferRays = 128;
 constexpr int kLdsFrags = 64;  // 64 KB
 
 struct InferArgs {
-    const float *feat;
-    int64_t ldf, n;
+    const float *feat;    // element (ray r, feature f) at feat[r * srow + f * scol]
+    int64_t srow, scol, n;
     int nfeat, nout;
     const float *mu, *isd;
     const __bf16 *w;
@@ -328,14 +328,14 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs a) {
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int64_t ray = ray0 + 16 * t + r;
-        const float *row = a.feat + ray * a.ldf;
+        const float *row = a.feat + ray * a.srow;
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
             for (int j = 0; j < 8; ++j) {
                 const int f = 32 * c + 8 * g + j;
                 float v = 0.f;
-                if (ray < a.n && f < a.nfeat) v = (row[f] - a.mu[f]) * a.isd[f];
+                if (ray < a.n && f < a.nfeat) v = (row[f * a.scol] - a.mu[f]) * a.isd[f];
                 x0[c][t][j] = (__bf16)v;
             }
     }
@@ -451,23 +451,24 @@ extern "C" int tt_adam(float *w, const float *g, float *m, float *v, void *w_bf1
     return TT_OK;
 }
 
-extern "C" int tt_standardize_bf16(const float *x, int64_t ldx, int64_t rows, int32_t cols, const float *mu,
-                                   const float *inv_sd, void *out_bf16, int64_t ldo, void *stream) {
-    TT_REQUIRE(x && mu && inv_sd && out_bf16 && rows >= 0 && cols > 0 && ldo >= cols && ldx >= cols,
+extern "C" int tt_standardize_bf16(const float *x, int64_t srow, int64_t scol, int64_t rows, int32_t cols,
+                                   const float *mu, const float *inv_sd, void *out_bf16, int64_t ldo, void *stream) {
+    TT_REQUIRE(x && mu && inv_sd && out_bf16 && rows >= 0 && cols > 0 && ldo >= cols && srow >= 1 && scol >= 1,
                "tt_standardize_bf16: bad arguments");
     if (rows == 0) return TT_OK;
     const int64_t n = rows * ldo;
     hipLaunchKernelGGL(standardize_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream),
-                       x, ldx, rows, cols, mu, inv_sd, static_cast<__bf16 *>(out_bf16), ldo);
+                       x, srow, scol, rows, cols, mu, inv_sd, static_cast<__bf16 *>(out_bf16), ldo);
     TT_LAUNCH_CHECK("tt_standardize_bf16 launch");
     return TT_OK;
 }
 
-extern "C" int tt_mlp_infer(const float *feat, int64_t ldf, int64_t n, int32_t nfeat, const float *mu,
+extern "C" int tt_mlp_infer(const float *feat, int64_t srow, int64_t scol, int64_t n, int32_t nfeat, const float *mu,
                             const float *inv_sd, const void *w_packed, const float *bias_packed, float *Y, int64_t ldy,
                             int32_t nout, void *stream) {
     TT_REQUIRE(feat && mu && inv_sd && w_packed && bias_packed && Y, "tt_mlp_infer: null pointer");
-    TT_REQUIRE(nfeat > 0 && nfeat <= 64 && ldf >= nfeat, "tt_mlp_infer: need 0 < nfeat <= 64 and ldf >= nfeat");
+    TT_REQUIRE(nfeat > 0 && nfeat <= 64, "tt_mlp_infer: need 0 < nfeat <= 64");
+    TT_REQUIRE(srow >= 1 && scol >= 1, "tt_mlp_infer: strides must be >= 1");
     TT_REQUIRE(nout > 0 && nout <= 16 && ldy >= nout, "tt_mlp_infer: need 0 < nout <= 16 and ldy >= nout");
     TT_REQUIRE(((uintptr_t)w_packed & 15) == 0 && ((uintptr_t)bias_packed & 15) == 0,
                "tt_mlp_infer: packed weights and biases must be 16-byte aligned");
@@ -475,7 +476,8 @@ extern "C" int tt_mlp_infer(const float *feat, int64_t ldf, int64_t n, int32_t n
     if (n == 0) return TT_OK;
     InferArgs a;
     a.feat = feat;
-    a.ldf = ldf;
+    a.srow = srow;
+    a.scol = scol;
     a.n = n;
     a.nfeat = nfeat;
     a.nout = nout;

@@ -3,9 +3,9 @@
 // Not a reference function: the cheap estimate whose residual against the
 // full trace (tt_trace_rk4, the work of particle_tracker.py:312-380) the
 // surrogate MLP learns.  The field is integrated along straight lines over
-// z-blocks of k cells: block_kernel stores, per block and node column, the
+// z-blocks of k cells: block_kernel stores, per node column and block, the
 // exact z-integral G of the trilinear field (the sum of the cells' mid-plane
-// values) and its first moment M about the block centre.  Each ray then takes
+// values) and its first moment M about the block centre, column by column.  Each ray then takes
 // one drift-kick-drift step per block: drift half a block along its current
 // direction, kick by G at that position, drift the second half.  Two
 // first-moment terms make the straight-line integral follow the ray: the
@@ -18,7 +18,8 @@
 // four stages); this takes one per k cells.
 //
 // Outputs per ray: base = [x, theta, y, phi, phase] (float64, the rf layout
-// plus phase) and 48 float32 features: 16 z-segment sums of the x kicks and of
+// plus phase) and 48 float32 features, stored feature-major (feature q of ray
+// r at feat[q * ldf + r], so every store of a wave is one contiguous run): 16 z-segment sums of the x kicks and of
 // the y kicks (a step belongs to the segment of its first cell, 16 c / ncell),
 // then U - U0, V - V0, W - W0, the exit displacement from the entry line (x,
 // y, cells), the phase, U0, V0, W0 - 1, the entry-cell fractions in x and y,
@@ -33,7 +34,7 @@ constexpr int kSeg = 16;
 
 struct PathArgs {
     const float4 *field;  // node field [nz][ny][nx]
-    const float4 *G, *M;  // block integrals and first moments [nb][ny][nx]
+    const float4 *GM;     // per column and block: {G, M}, [ny][nx][nb][2] float4
     int k, nb;
     int nx, ny, nz;
     double x0, y0, z0, dx, dy, dz;
@@ -62,6 +63,17 @@ __device__ __forceinline__ void load_corners(const float4 *p, int nx, float4 *c)
     c[1] = p[1];
     c[2] = p[nx];
     c[3] = p[nx + 1];
+}
+
+// atan for the small exit angles of a near-axial beam: below 1e-2 the
+// series to x^7 is exact to double rounding (the next term, x^9 / 9, is
+// below 1e-17 x); larger angles take the library atan.
+__device__ __forceinline__ double small_atan(double x) {
+    if (fabs(x) < 1e-2) {
+        const double x2 = x * x;
+        return x * fma(x2, fma(x2, fma(x2, -1.0 / 7.0, 1.0 / 5.0), -1.0 / 3.0), 1.0);
+    }
+    return atan(x);
 }
 
 struct PathState {
@@ -113,8 +125,8 @@ __device__ __forceinline__ void kick_drift(const PathArgs &a, const Mid &m, floa
     S.dpy = fmaf(fmaf(hm, S.V, -mvy) * a.sy, iW1, m.dmy);
     // z-segment sums: segment q holds cells [ceil(q ncell / 16), ceil((q + 1) ncell / 16))
     while (c0 >= S.seg_end) {
-        f[S.seg] = S.segx;
-        f[kSeg + S.seg] = S.segy;
+        f[S.seg * a.ldf] = S.segx;
+        f[(kSeg + S.seg) * a.ldf] = S.segy;
         S.segx = S.segy = 0.f;
         ++S.seg;
         S.seg_end = ((S.seg + 1) * ncell + kSeg - 1) / kSeg;
@@ -136,12 +148,12 @@ __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
     const double *s = a.s0 + r;
     const double x = s[0], y = s[a.ld], z = s[2 * a.ld];
     const double U0 = s[3 * a.ld] / tt::kC, V0 = s[4 * a.ld] / tt::kC, W0 = s[5 * a.ld] / tt::kC;
-    float *f = a.feat + r * a.ldf;
+    float *f = a.feat + r;  // feature-major: feature q of this ray at f[q * ldf]
     double *b = a.base + r;
     const double nanv = __longlong_as_double(0x7ff8000000000000ll);
     if (!(W0 > 0.0) || isnan(x) || isnan(y) || isnan(z) || isnan(U0) || isnan(V0)) {
         for (int c = 0; c < 5; ++c) b[c * a.ldb] = nanv;
-        for (int c = 0; c < 48; ++c) f[c] = 0.f;
+        for (int c = 0; c < 48; ++c) f[c * a.ldf] = 0.f;
         return;
     }
     double px0 = (x - a.x0) / a.dx, py0 = (y - a.y0) / a.dy, Zs = (z - a.z0) / a.dz;
@@ -160,7 +172,7 @@ __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
     S.ph = 0.0;
     S.segx = S.segy = S.crossx = S.crossy = S.qx = S.qy = 0.f;
     S.cix = S.ciy = -1;
-    for (int q = 0; q < 2 * kSeg; ++q) f[q] = 0.f;
+    for (int q = 0; q < 2 * kSeg; ++q) f[q * a.ldf] = 0.f;
     const int iz0 = Zs >= (double)ncell ? ncell : (int)Zs;
     S.seg = (iz0 * kSeg) / (ncell > 0 ? ncell : 1);
     S.seg_end = ((S.seg + 1) * ncell + kSeg - 1) / kSeg;
@@ -181,16 +193,23 @@ __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
         g = m.inb ? make_float4(w * g.x, w * g.y, w * g.z, w * g.w) : make_float4(0.f, 0.f, 0.f, 0.f);
         kick_drift(a, m, g, w, 0.f, 0.f, c, ncell, S, f);
     }
-    // z-blocks
-    const int64_t bplane = (int64_t)a.nx * a.ny;
+    // z-blocks: a column's blocks are consecutive in memory, so a wave's
+    // successive steps read neighbouring bytes of the same few columns
+    const int64_t cstride = 2 * (int64_t)a.nb;  // float4 per column
     for (int bb = bstart / a.k; bb < a.nb && bstart < ncell; ++bb) {
         const int b0 = bb * a.k;
         const float w = (float)(min(ncell, b0 + a.k) - b0);
         const Mid m = half_drift(a, S, pxb, pyb, w);
-        const int64_t o = bb * bplane + (int64_t)m.iy * a.nx + m.ix;
+        const float4 *q = a.GM + ((int64_t)m.iy * a.nx + m.ix) * cstride + 2 * bb;
         float4 Gc[4], Mc[4];
-        load_corners(a.G + o, a.nx, Gc);
-        load_corners(a.M + o, a.nx, Mc);
+        Gc[0] = q[0];
+        Mc[0] = q[1];
+        Gc[1] = q[cstride];
+        Mc[1] = q[cstride + 1];
+        Gc[2] = q[a.nx * cstride];
+        Mc[2] = q[a.nx * cstride + 1];
+        Gc[3] = q[(a.nx + 1) * cstride];
+        Mc[3] = q[(a.nx + 1) * cstride + 1];
         const float4 G = bilerp(Gc, m.tx, m.ty);
         const float4 Mdx = lerp4(sub4(Mc[1], Mc[0]), sub4(Mc[3], Mc[2]), m.ty);
         const float4 Mdy = lerp4(sub4(Mc[2], Mc[0]), sub4(Mc[3], Mc[1]), m.tx);
@@ -216,82 +235,88 @@ __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
         kick_drift(a, m, g, w, m.inb ? mvx : 0.f, m.inb ? mvy : 0.f, b0, ncell, S, f);
     }
     if (S.seg < kSeg) {
-        f[S.seg] = S.segx;
-        f[kSeg + S.seg] = S.segy;
+        f[S.seg * a.ldf] = S.segx;
+        f[(kSeg + S.seg) * a.ldf] = S.segy;
     }
     const float U = S.U, V = S.V, Wd = S.Wd, dpx = S.dpx, dpy = S.dpy;
     const double ph = S.ph;
     const double Uf = U, Vf = V, Wf = 1.0 + (double)Wd;
     const double pxe = px0 + (double)dpx, pye = py0 + (double)dpy;
     b[0] = a.x0 + pxe * a.dx;
-    b[a.ldb] = atan(Uf / Wf);
+    b[a.ldb] = small_atan(Uf / Wf);
     b[2 * a.ldb] = a.y0 + pye * a.dy;
-    b[3 * a.ldb] = atan(Vf / Wf);
+    b[3 * a.ldb] = small_atan(Vf / Wf);
     b[4 * a.ldb] = ph;
     const double L = (double)ncell - Zs;
-    float *gf = f + 2 * kSeg;
-    gf[0] = (float)(Uf - U0);
-    gf[1] = (float)(Vf - V0);
-    gf[2] = Wd - (float)(W0 - 1.0);
-    gf[3] = (float)(pxe - (px0 + (double)a.sx * U0 / W0 * L));
-    gf[4] = (float)(pye - (py0 + (double)a.sy * V0 / W0 * L));
-    gf[5] = (float)ph;
-    gf[6] = (float)U0;
-    gf[7] = (float)V0;
-    gf[8] = (float)(W0 - 1.0);
-    gf[9] = (float)(px0 - floor(px0));
-    gf[10] = (float)(py0 - floor(py0));
-    gf[11] = S.crossx;
-    gf[12] = S.crossy;
-    gf[13] = S.qx;
-    gf[14] = S.qy;
-    gf[15] = (float)L;
+    float *gf = f + 2 * kSeg * a.ldf;
+    const int64_t L1 = a.ldf;
+    gf[0 * L1] = (float)(Uf - U0);
+    gf[1 * L1] = (float)(Vf - V0);
+    gf[2 * L1] = Wd - (float)(W0 - 1.0);
+    gf[3 * L1] = (float)(pxe - (px0 + (double)a.sx * U0 / W0 * L));
+    gf[4 * L1] = (float)(pye - (py0 + (double)a.sy * V0 / W0 * L));
+    gf[5 * L1] = (float)ph;
+    gf[6 * L1] = (float)U0;
+    gf[7 * L1] = (float)V0;
+    gf[8 * L1] = (float)(W0 - 1.0);
+    gf[9 * L1] = (float)(px0 - floor(px0));
+    gf[10 * L1] = (float)(py0 - floor(py0));
+    gf[11 * L1] = S.crossx;
+    gf[12 * L1] = S.crossy;
+    gf[13 * L1] = S.qx;
+    gf[14 * L1] = S.qy;
+    gf[15 * L1] = (float)L;
 }
 
-// Block integrals: thread per (column, block); G = sum_c (f[c] + f[c+1]) / 2,
-// M = sum_c (c + 1/2 - zmid) (f[c] + f[c+1]) / 2 + (f[b1] - f[b0]) / 12 over
-// the block's cells c in [b0, b1).
-__global__ void block_kernel(const float4 *__restrict__ f, float4 *__restrict__ G, float4 *__restrict__ M,
-                             int64_t plane, int ncell, int k, int nb) {
+// Block integrals: thread per node column, marching down z once; for each
+// block of cells [b0, b1): G = sum_c (f[c] + f[c+1]) / 2 and
+// M = sum_c (c + 1/2 - zmid) (f[c] + f[c+1]) / 2 + (f[b1] - f[b0]) / 12,
+// stored as GM[col][b] = {G, M}.
+__global__ void block_kernel(const float4 *__restrict__ f, float4 *__restrict__ GM, int64_t plane, int ncell, int k,
+                             int nb) {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int bb = blockIdx.y;
-    if (col >= plane || bb >= nb) return;
-    const int b0 = bb * k, b1 = min(ncell, b0 + k);
-    const float zm = 0.5f * (float)(b0 + b1);
-    float4 lo = f[(int64_t)b0 * plane + col];
-    const float4 first = lo;
-    float4 g = make_float4(0.f, 0.f, 0.f, 0.f), m = g;
-    for (int c = b0; c < b1; ++c) {
-        const float4 hi = f[(int64_t)(c + 1) * plane + col];
-        const float4 mid = make_float4(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z),
-                                       0.5f * (lo.w + hi.w));
-        const float zc = (float)c + 0.5f - zm;
-        g.x += mid.x;
-        g.y += mid.y;
-        g.z += mid.z;
-        g.w += mid.w;
-        m.x = fmaf(zc, mid.x, m.x);
-        m.y = fmaf(zc, mid.y, m.y);
-        m.z = fmaf(zc, mid.z, m.z);
-        m.w = fmaf(zc, mid.w, m.w);
-        lo = hi;
+    if (col >= plane) return;
+    float4 *out = GM + col * 2 * nb;
+    float4 lo = f[col];
+    for (int bb = 0; bb < nb; ++bb) {
+        const int b0 = bb * k, b1 = min(ncell, b0 + k);
+        const float zm = 0.5f * (float)(b0 + b1);
+        const float4 first = lo;
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f), m = g;
+        for (int c = b0; c < b1; ++c) {
+            const float4 hi = f[(int64_t)(c + 1) * plane + col];
+            const float4 mid = make_float4(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z),
+                                           0.5f * (lo.w + hi.w));
+            const float zc = (float)c + 0.5f - zm;
+            g.x += mid.x;
+            g.y += mid.y;
+            g.z += mid.z;
+            g.w += mid.w;
+            m.x = fmaf(zc, mid.x, m.x);
+            m.y = fmaf(zc, mid.y, m.y);
+            m.z = fmaf(zc, mid.z, m.z);
+            m.w = fmaf(zc, mid.w, m.w);
+            lo = hi;
+        }
+        const float s12 = 1.0f / 12.0f;
+        m.x = fmaf(lo.x - first.x, s12, m.x);
+        m.y = fmaf(lo.y - first.y, s12, m.y);
+        m.z = fmaf(lo.z - first.z, s12, m.z);
+        m.w = fmaf(lo.w - first.w, s12, m.w);
+        out[2 * bb] = g;
+        out[2 * bb + 1] = m;
     }
-    const float s12 = 1.0f / 12.0f;
-    m.x = fmaf(lo.x - first.x, s12, m.x);
-    m.y = fmaf(lo.y - first.y, s12, m.y);
-    m.z = fmaf(lo.z - first.z, s12, m.z);
-    m.w = fmaf(lo.w - first.w, s12, m.w);
-    G[(int64_t)bb * plane + col] = g;
-    M[(int64_t)bb * plane + col] = m;
 }
 
 // corrected = base + Y[:, :5] * scale; flag when a predicted error bound
 // (|Y[:, 5]| * e_pos for position, |Y[:, 6]| * e_ang for angle) exceeds its
-// threshold, or when the estimate is not finite.  Flagged ray indices are
+// threshold, or when the estimate is not finite.  Flagged ray indices, plus
+// `first` (the index of this batch's first ray in a larger ray set), are
 // appended to list (count in *nflag).
 __global__ void apply_kernel(const double *base, int64_t ldb, const float *Y, int64_t ldy, int64_t n, float s_pos,
                              float s_ang, float s_ph, float e_pos, float e_ang, double t_pos, double t_ang,
-                             double *rf, int64_t ldr, double *phase, int32_t *flag, int32_t *list, int32_t *nflag) {
+                             double *rf, int64_t ldr, double *phase, int32_t *flag, int32_t *list, int32_t *nflag,
+                             int64_t first) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float *y = Y + i * ldy;
@@ -308,53 +333,53 @@ __global__ void apply_kernel(const double *base, int64_t ldb, const float *Y, in
     const double ep = fabs((double)(y[5] * e_pos)), ea = fabs((double)(y[6] * e_ang));
     const bool bad = !(ep <= t_pos && ea <= t_ang) || isnan(x) || isnan(th) || isnan(yy) || isnan(ph);
     if (flag) flag[i] = bad ? 1 : 0;
-    if (bad && list) list[atomicAdd(nflag, 1)] = (int32_t)i;
+    if (bad && list) list[atomicAdd(nflag, 1)] = (int32_t)(first + i);
 }
 
 }  // namespace
 
 extern "C" int tt_surrogate_apply(const double *base, int64_t ldb, const float *Y, int64_t ldy, int64_t n,
                                   const float *scales, const double *thresholds, double *rf, int64_t ldr,
-                                  double *phase, int32_t *flag, int32_t *list, int32_t *nflag, void *stream) {
+                                  double *phase, int32_t *flag, int32_t *list, int32_t *nflag, int64_t first,
+                                  void *stream) {
     TT_REQUIRE(base && Y && scales && thresholds && rf, "tt_surrogate_apply: null pointer");
     TT_REQUIRE(n >= 0 && ldb >= n && ldr >= n && ldy >= 7, "tt_surrogate_apply: bad sizes");
     TT_REQUIRE(!list || nflag, "tt_surrogate_apply: list needs nflag");
+    TT_REQUIRE(first >= 0 && first + n <= 0x7fffffff, "tt_surrogate_apply: ray indices must fit int32");
     if (n == 0) return TT_OK;
     hipLaunchKernelGGL(apply_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream), base,
                        ldb, Y, ldy, n, scales[0], scales[1], scales[2], scales[3], scales[4], thresholds[0],
-                       thresholds[1], rf, ldr, phase, flag, list, nflag);
+                       thresholds[1], rf, ldr, phase, flag, list, nflag, first);
     TT_LAUNCH_CHECK("tt_surrogate_apply launch");
     return TT_OK;
 }
 
-extern "C" int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t k, float *G4, float *M4,
-                               void *stream) {
-    TT_REQUIRE(field4 && grid && G4 && M4, "tt_field_blocks: null pointer");
+extern "C" int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t k, float *GM4, void *stream) {
+    TT_REQUIRE(field4 && grid && GM4, "tt_field_blocks: null pointer");
     TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "tt_field_blocks: grid needs >= 2 nodes per axis");
     TT_REQUIRE(k >= 1 && k <= 1024, "tt_field_blocks: block size k must be in [1, 1024]");
     const int ncell = grid->nz - 1, nb = (ncell + k - 1) / k;
     TT_REQUIRE(nb <= 65535, "tt_field_blocks: too many blocks");
     const int64_t plane = (int64_t)grid->nx * grid->ny;
-    dim3 g((unsigned)tt::ceil_div(plane, 256), (unsigned)nb);
-    hipLaunchKernelGGL(block_kernel, g, dim3(256), 0, tt::as_stream(stream), reinterpret_cast<const float4 *>(field4),
-                       reinterpret_cast<float4 *>(G4), reinterpret_cast<float4 *>(M4), plane, ncell, (int)k, nb);
+    hipLaunchKernelGGL(block_kernel, dim3((unsigned)tt::ceil_div(plane, 256)), dim3(256), 0, tt::as_stream(stream),
+                       reinterpret_cast<const float4 *>(field4), reinterpret_cast<float4 *>(GM4), plane, ncell, (int)k,
+                       nb);
     TT_LAUNCH_CHECK("tt_field_blocks launch");
     return TT_OK;
 }
 
-extern "C" int tt_path_features(const float *field4, const float *G4, const float *M4, int32_t k, const tt_grid *grid,
-                                const double *s0, int64_t ld, int64_t n, const int32_t *order, float *feat,
-                                int64_t ldf, double *base, int64_t ldb, void *stream) {
-    TT_REQUIRE(field4 && G4 && M4 && grid && s0 && feat && base, "tt_path_features: null pointer");
-    TT_REQUIRE(n >= 0 && ld >= n && ldb >= n && ldf >= 48, "tt_path_features: bad sizes");
+extern "C" int tt_path_features(const float *field4, const float *GM4, int32_t k, const tt_grid *grid, const double *s0,
+                                int64_t ld, int64_t n, const int32_t *order, float *feat, int64_t ldf, double *base,
+                                int64_t ldb, void *stream) {
+    TT_REQUIRE(field4 && GM4 && grid && s0 && feat && base, "tt_path_features: null pointer");
+    TT_REQUIRE(n >= 0 && ld >= n && ldb >= n && ldf >= n, "tt_path_features: bad sizes");
     TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "tt_path_features: grid needs >= 2 nodes per axis");
     TT_REQUIRE(grid->dx > 0 && grid->dy > 0 && grid->dz > 0, "tt_path_features: spacing must be > 0");
     TT_REQUIRE(k >= 1 && k <= 1024, "tt_path_features: block size k must be in [1, 1024]");
     if (n == 0) return TT_OK;
     PathArgs a;
     a.field = reinterpret_cast<const float4 *>(field4);
-    a.G = reinterpret_cast<const float4 *>(G4);
-    a.M = reinterpret_cast<const float4 *>(M4);
+    a.GM = reinterpret_cast<const float4 *>(GM4);
     a.k = k;
     a.nb = (grid->nz - 1 + k - 1) / k;
     a.nx = grid->nx;

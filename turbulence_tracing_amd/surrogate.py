@@ -170,12 +170,10 @@ class MLP:
         """Fused forward pass on fp32 features [n, dims[0]] (standardised with
         mu / isd inside the kernel): fp32 [n, dims[-1]]."""
         n = feat.shape[0]
-        if feat.stride(1) != 1:
-            raise ValueError("infer: features must be row-contiguous")
         w, b = self.packed()
         Y = out if out is not None else torch.empty((n, self.dims[-1]), dtype=torch.float32, device=self.device)
-        self.lib.tt_mlp_infer(_p(feat), feat.stride(0), n, self.dims[0], _p(mu), _p(isd), _p(w), _p(b), _p(Y),
-                              Y.stride(0), self.dims[-1], _lib.stream_ptr())
+        self.lib.tt_mlp_infer(_p(feat), feat.stride(0), feat.stride(1), n, self.dims[0], _p(mu), _p(isd), _p(w),
+                              _p(b), _p(Y), Y.stride(0), self.dims[-1], _lib.stream_ptr())
         return Y
 
     def state(self):
@@ -216,33 +214,39 @@ class Dataset:
 
 
 def block_fields(field4, grid, k=16):
-    """Straight-line z-block integrals (G) and first moments (M) of the node
-    field over blocks of k cells, each [ceil((nz - 1) / k), ny, nx, 4]."""
+    """Straight-line z-block integrals G and first moments M of the node field
+    over blocks of k cells, column by column: [ny, nx, nb, 2, 4] float32
+    ({G, M} per column and block), nb = ceil((nz - 1) / k)."""
     lib = _lib.load()
     nb = -(-(grid.nz - 1) // k)
-    shape = (nb, grid.ny, grid.nx, 4)
-    G = torch.empty(shape, dtype=torch.float32, device=field4.device)
-    M = torch.empty(shape, dtype=torch.float32, device=field4.device)
-    lib.tt_field_blocks(_p(field4), C.byref(grid), k, _p(G), _p(M), _lib.stream_ptr())
-    return G, M
+    GM = torch.empty((grid.ny, grid.nx, nb, 2, 4), dtype=torch.float32, device=field4.device)
+    lib.tt_field_blocks(_p(field4), C.byref(grid), k, _p(GM), _lib.stream_ptr())
+    return GM
 
 
-def path_estimate(field4, grid, s0, n=None, k=16, blocks=None, order=None):
-    """Cheap path estimate base [5, n] (fp64) and features [n, 48] (fp32)
-    over z-blocks of k cells.  ``blocks`` = block_fields(field4, grid, k)
+def path_estimate(field4, grid, s0, n=None, k=16, blocks=None, order=None, base=None, feat=None):
+    """Cheap path estimate base [5, n] (fp64) and features [n, 48] (fp32,
+    a feature-major view: strides (1, n)) over z-blocks of k cells.  ``blocks`` = block_fields(field4, grid, k)
     (built here when not given); ``order`` (int32, optional) is the
-    processing order of the rays."""
+    processing order of the rays; ``base`` / ``feat`` (optional) receive the
+    results (contiguous, of those shapes)."""
     lib = _lib.load()
     n = s0.shape[1] if n is None else n
     ld = s0.stride(0)
     dev = s0.device
-    G, M = blocks if blocks is not None else block_fields(field4, grid, k)
-    if G.shape[0] != -(-(grid.nz - 1) // k):
+    GM = blocks if blocks is not None else block_fields(field4, grid, k)
+    if GM.shape[2] != -(-(grid.nz - 1) // k):
         raise ValueError(f"path_estimate: blocks were built for another block size than k={k}")
-    base = torch.empty((5, n), dtype=torch.float64, device=dev)
-    feat = torch.empty((n, NFEAT), dtype=torch.float32, device=dev)
-    lib.tt_path_features(_p(field4), _p(G), _p(M), k, C.byref(grid), _p(s0), ld, n, _p(order), _p(feat), NFEAT,
-                         _p(base), n, _lib.stream_ptr())
+    if base is None:
+        base = torch.empty((5, n), dtype=torch.float64, device=dev)
+    if feat is None:
+        feat = torch.empty((NFEAT, n), dtype=torch.float32, device=dev).t()
+    if tuple(base.shape) != (5, n) or not base.is_contiguous():
+        raise ValueError("path_estimate: base must be a contiguous [5, n] tensor")
+    if tuple(feat.shape) != (n, NFEAT) or feat.stride(0) != 1 or feat.stride(1) < max(n, 1):
+        raise ValueError("path_estimate: feat must be a feature-major [n, 48] view (strides (1, >= n))")
+    lib.tt_path_features(_p(field4), _p(GM), k, C.byref(grid), _p(s0), ld, n, _p(order), _p(feat),
+                         feat.stride(1), _p(base), n, _lib.stream_ptr())
     return base, feat
 
 
@@ -333,8 +337,8 @@ class Surrogate:
     def standardize(self, feat):
         n = feat.shape[0]
         X = torch.empty((n, NFEAT), dtype=torch.bfloat16, device=self.device)
-        self.lib.tt_standardize_bf16(_p(feat), NFEAT, n, NFEAT, _p(self.mu), _p(self.isd), _p(X), NFEAT,
-                                     _lib.stream_ptr())
+        self.lib.tt_standardize_bf16(_p(feat), feat.stride(0), feat.stride(1), n, NFEAT, _p(self.mu), _p(self.isd),
+                                     _p(X), NFEAT, _lib.stream_ptr())
         return X
 
     def train(self, feat, target, epochs=4, batch=8192, lr=2e-3, lr_min=5e-5, wd=0.0, log=None, seed=0):
@@ -373,54 +377,108 @@ class Surrogate:
         """Network output [n, 8] for fp32 features [n, 48] (fused kernel)."""
         return self.mlp.infer(feat, self.mu, self.isd, out=out)
 
-    def apply(self, base, Y, t_pos=1e-6, t_ang=1e-6, want_list=True):
+    def apply(self, base, Y, t_pos=1e-6, t_ang=1e-6, want_list=True, out=None, first=0):
+        """Corrected exit state and fallback flags.  ``out`` = (rf, phase,
+        flag, list, count) views to write into (rf may be a column slice of a
+        larger 4 x N tensor); ``first`` is added to the listed ray indices."""
         n = base.shape[1]
-        rf = torch.empty((4, n), dtype=torch.float64, device=self.device)
-        phase = torch.empty(n, dtype=torch.float64, device=self.device)
-        flag = torch.empty(n, dtype=torch.int32, device=self.device)
-        lst = torch.empty(n, dtype=torch.int32, device=self.device) if want_list else None
-        cnt = torch.zeros(1, dtype=torch.int32, device=self.device)
+        if out is None:
+            rf = torch.empty((4, n), dtype=torch.float64, device=self.device)
+            phase = torch.empty(n, dtype=torch.float64, device=self.device)
+            flag = torch.empty(n, dtype=torch.int32, device=self.device)
+            lst = torch.empty(n, dtype=torch.int32, device=self.device) if want_list else None
+            cnt = torch.zeros(1, dtype=torch.int32, device=self.device)
+        else:
+            rf, phase, flag, lst, cnt = out
         sc = (C.c_float * 5)(self.norm.s_pos, self.norm.s_ang, self.norm.s_ph, self.norm.e_pos, self.norm.e_ang)
         th = (C.c_double * 2)(t_pos, t_ang)
-        self.lib.tt_surrogate_apply(_p(base), n, _p(Y), NOUT, n, sc, th, _p(rf), n, _p(phase), _p(flag), _p(lst),
-                                    _p(cnt), _lib.stream_ptr())
+        self.lib.tt_surrogate_apply(_p(base), base.stride(0), _p(Y), Y.stride(0), n, sc, th, _p(rf), rf.stride(0),
+                                    _p(phase), _p(flag), _p(lst), _p(cnt), int(first), _lib.stream_ptr())
         return rf, phase, flag, lst, cnt
 
-    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, blocks=None, timings=None):
+    def _side_streams(self):
+        if getattr(self, "_streams", None) is None:
+            self._streams = (torch.cuda.Stream(device=self.device), torch.cuda.Stream(device=self.device))
+        return self._streams
+
+    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, blocks=None, timings=None, chunks=4):
         """Surrogate + fallback trace of rays s0 (6 x n, device): the block
         path estimate, the network's correction, and the full RK4 tracer on
         the rays the network flags (predicted position error above t_pos
         metres or angle error above t_ang radians, or a non-finite
         estimate).  Returns rf (4 x n), phase (n) and the flag per ray
-        (1 = re-traced).  ``timings`` (a dict) receives per-stage ms."""
+        (1 = re-traced).
+
+        The rays are cut into ``chunks`` batches: the path estimate of batch
+        i + 1 runs on one stream while the network and the correction of
+        batch i run on another, so the VALU-bound estimate and the
+        MFMA-bound network share the CUs.  ``timings`` (a dict) receives the
+        total and fallback ms (and per-stage ms when chunks == 1)."""
         n = s0.shape[1]
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)] if timings is not None else None
-        if ev:
-            ev[0].record()
+        dev = self.device
+        main = torch.cuda.current_stream()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t0.record(main)
         if blocks is None:
             blocks = block_fields(field4, grid, self.k)
-        if ev:
-            ev[1].record()
-        base, feat = path_estimate(field4, grid, s0, k=self.k, blocks=blocks)
-        if ev:
-            ev[2].record()
-        Y = self.predict(feat)
-        if ev:
-            ev[3].record()
-        rf, phase, flag, lst, cnt = self.apply(base, Y, t_pos, t_ang)
+        rf = torch.empty((4, n), dtype=torch.float64, device=dev)
+        phase = torch.empty(n, dtype=torch.float64, device=dev)
+        flag = torch.empty(n, dtype=torch.int32, device=dev)
+        lst = torch.empty(max(n, 1), dtype=torch.int32, device=dev)
+        cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+        chunks = max(1, min(int(chunks), -(-n // 4096)))
+        step = -(-(-(-n // chunks)) // 256) * 256
+        bounds = [(a, min(n, a + step)) for a in range(0, n, step)]
+        bases = [torch.empty((5, b - a), dtype=torch.float64, device=dev) for a, b in bounds]
+        feats = [torch.empty((NFEAT, b - a), dtype=torch.float32, device=dev).t() for a, b in bounds]
+        Ys = [torch.empty((b - a, NOUT), dtype=torch.float32, device=dev) for a, b in bounds]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)] if (timings is not None and
+                                                                          len(bounds) == 1) else None
+        if len(bounds) == 1:
+            if ev:
+                ev[0].record(main)
+            path_estimate(field4, grid, s0, k=self.k, blocks=blocks, base=bases[0], feat=feats[0])
+            if ev:
+                ev[1].record(main)
+            self.predict(feats[0], out=Ys[0])
+            if ev:
+                ev[2].record(main)
+            self.apply(bases[0], Ys[0], t_pos, t_ang, out=(rf, phase, flag, lst, cnt))
+            if ev:
+                ev[3].record(main)
+        else:
+            sp, sm = self._side_streams()
+            ready = torch.cuda.Event()
+            ready.record(main)
+            sp.wait_event(ready)
+            sm.wait_event(ready)
+            for (a, b), base, feat, Y in zip(bounds, bases, feats, Ys):
+                with torch.cuda.stream(sp):
+                    path_estimate(field4, grid, s0[:, a:b], k=self.k, blocks=blocks, base=base, feat=feat)
+                    done = torch.cuda.Event()
+                    done.record(sp)
+                with torch.cuda.stream(sm):
+                    sm.wait_event(done)
+                    self.predict(feat, out=Y)
+                    self.apply(base, Y, t_pos, t_ang, out=(rf[:, a:b], phase[a:b], flag[a:b], lst, cnt), first=a)
+            main.wait_stream(sp)
+            main.wait_stream(sm)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t1.record(main)
         nf = int(cnt.item())
-        if ev:
-            ev[4].record()
         if nf > 0:
             opts = _lib.TraceOpts(h, 0.0, 0)
             self.lib.tt_trace_rk4(_p(field4), C.byref(grid), C.byref(opts), _p(s0), s0.stride(0), nf, _p(lst),
                                   _p(rf), n, _p(phase), None, n, None, _lib.stream_ptr())
-        if ev:
-            ev[5].record()
+        if timings is not None:
+            t2 = torch.cuda.Event(enable_timing=True)
+            t2.record(main)
             torch.cuda.synchronize()
-            timings.update(blocks=ev[0].elapsed_time(ev[1]), path=ev[1].elapsed_time(ev[2]),
-                           mlp=ev[2].elapsed_time(ev[3]), apply=ev[3].elapsed_time(ev[4]),
-                           fallback=ev[4].elapsed_time(ev[5]), n_fallback=nf)
+            timings.update(estimate_and_network=t0.elapsed_time(t1), fallback=t1.elapsed_time(t2),
+                           total=t0.elapsed_time(t2), n_fallback=nf, chunks=len(bounds))
+            if ev:
+                timings.update(path=ev[0].elapsed_time(ev[1]), mlp=ev[1].elapsed_time(ev[2]),
+                               apply=ev[2].elapsed_time(ev[3]))
         return rf, phase, flag
 
     def save(self, path):
