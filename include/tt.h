@@ -178,6 +178,12 @@ int tt_beam_sample(int64_t n, double beam_size, double divergence, double extent
 int tt_ray_sort_scratch_bytes(const tt_grid *grid, int32_t brick_shift, size_t *bytes);
 int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
                 void *scratch, int32_t *order, void *stream);
+/* The same ordering, also moving the rays: out (6 x ldo float64) = s0's
+ * columns in `order`.  staging (device, 8 n float64, 16-byte aligned) holds
+ * one 64-byte record per ray in between, so the scattered writes are whole
+ * records and the final rows are written coalesced. */
+int tt_ray_sort_rays(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
+                     void *scratch, int32_t *order, double *staging, double *out, int64_t ldo, void *stream);
 
 /* ------------------------------------------------------------- detectors
  * Replaces the ray_transfer_matrix optics and Rays.histogram

@@ -306,6 +306,38 @@ def test_ray_sort_is_permutation_by_column(lib):
     assert np.all(np.diff(keys) >= 0)
 
 
+@pytest.mark.parametrize("n,N", [(100000, 100000), (777, 1000), (1, 1)])
+def test_ray_sort_rays_moves_rays_exactly(lib, n, N):
+    """tt_ray_sort_rays writes s0's columns in the sort order (bit for bit),
+    for a full and a partial batch (out rows of length N > n)."""
+    import ctypes as C
+    import torch
+    from turbulence_tracing_amd import _lib
+    x = np.linspace(-5e-3, 5e-3, 65)
+    grid = _lib.Grid(65, 65, 65, -5e-3, -5e-3, -5e-3, x[1] - x[0], x[1] - x[0], x[1] - x[0])
+    rs = np.random.RandomState(n)
+    s0 = torch.as_tensor(np.concatenate([rs.uniform(-3e-3, 3e-3, (2, N)), rs.randn(4, N)]), device="cuda")
+    nbytes = C.c_size_t()
+    lib.tt_ray_sort_scratch_bytes(C.byref(grid), 0, C.byref(nbytes))
+    scratch = torch.empty(nbytes.value, dtype=torch.uint8, device="cuda")
+    order = torch.full((N,), -1, dtype=torch.int32, device="cuda")
+    staging = torch.empty((N, 8), dtype=torch.float64, device="cuda")
+    out = torch.full((6, N), 7.0, dtype=torch.float64, device="cuda")
+    lib.tt_ray_sort_rays(_lib.ptr(s0), N, n, C.byref(grid), 0, _lib.ptr(scratch), _lib.ptr(order), _lib.ptr(staging),
+                         _lib.ptr(out), N, _lib.stream_ptr())
+    o = order[:n].long().cpu().numpy()
+    assert np.array_equal(np.sort(o), np.arange(n))
+    np.testing.assert_array_equal(out[:, :n].cpu().numpy(), s0[:, :n].cpu().numpy()[:, o])
+    assert bool((out[:, n:] == 7.0).all())
+    # same order as tt_ray_sort's key order: entry-column keys non-decreasing
+    ix = np.floor((out[0, :n].cpu().numpy() + 5e-3) / (x[1] - x[0])).astype(np.int64)
+    iy = np.floor((out[1, :n].cpu().numpy() + 5e-3) / (x[1] - x[0])).astype(np.int64)
+    k = np.zeros_like(ix)
+    for bit in range(8):
+        k |= ((ix >> bit) & 1) << (2 * bit) | ((iy >> bit) & 1) << (2 * bit + 1)
+    assert np.all(np.diff(k) >= 0)
+
+
 # -------------------------------------------------------------- detectors
 @pytest.mark.parametrize("name", ["sh", "df", "lf", "afr"])
 def test_detectors_match_reference_golden(lib, golden, name):

@@ -132,6 +132,37 @@ __global__ __launch_bounds__(256) void scatter_kernel(const double *s0, int64_t 
     order[pos] = (int32_t)i;
 }
 
+// The scatter that also moves the rays: each ray's six values go, as one
+// 64-byte record, to its sorted position in a ray-major staging buffer
+// (three 16-byte stores into one aligned record instead of six scattered
+// 8-byte ones), and unpack_kernel then writes the sorted rows coalesced.
+__global__ __launch_bounds__(256) void scatter_rays_kernel(const double *s0, int64_t ld, int64_t n, SortGeom g,
+                                                           uint32_t *offsets, int32_t *order, double *staging) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = s0[i], y = s0[ld + i], z = s0[2 * ld + i];
+    const double vx = s0[3 * ld + i], vy = s0[4 * ld + i], vz = s0[5 * ld + i];
+    const uint32_t pos = atomicAdd(&offsets[ray_key(g, x, y)], 1u);
+    order[pos] = (int32_t)i;
+    double2 *rec = reinterpret_cast<double2 *>(staging + (int64_t)pos * 8);
+    rec[0] = make_double2(x, y);
+    rec[1] = make_double2(z, vx);
+    rec[2] = make_double2(vy, vz);
+}
+
+__global__ __launch_bounds__(256) void unpack_kernel(const double *staging, int64_t n, double *out, int64_t ldo) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 *rec = reinterpret_cast<const double2 *>(staging + i * 8);
+    const double2 a = rec[0], b = rec[1], c = rec[2];
+    out[i] = a.x;
+    out[ldo + i] = a.y;
+    out[2 * ldo + i] = b.x;
+    out[3 * ldo + i] = b.y;
+    out[4 * ldo + i] = c.x;
+    out[5 * ldo + i] = c.y;
+}
+
 int sort_geom(const tt_grid *grid, int shift, SortGeom &g, int64_t &nbins) {
     TT_REQUIRE(grid && grid->nx >= 1 && grid->ny >= 1, "tt_ray_sort: bad grid");
     TT_REQUIRE(shift >= 0 && shift <= 15, "tt_ray_sort: brick_shift must be in [0, 15]");
@@ -177,8 +208,9 @@ extern "C" int tt_ray_sort_scratch_bytes(const tt_grid *grid, int32_t brick_shif
     return TT_OK;
 }
 
-extern "C" int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
-                           void *scratch, int32_t *order, void *stream) {
+namespace {
+int sort_impl(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift, void *scratch,
+              int32_t *order, double *staging, double *out, int64_t ldo, void *stream) {
     SortGeom g;
     int64_t nbins;
     int rc = sort_geom(grid, brick_shift, g, nbins);
@@ -199,8 +231,28 @@ extern "C" int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_gri
     hipLaunchKernelGGL(scan_chunk, dim3(1), dim3(kScanBlock), 0, st, sums, chunks, sums2);
     hipLaunchKernelGGL(add_chunk, dim3((unsigned)chunks), dim3(kScanBlock), 0, st, hist, nbins,
                        (const uint32_t *)sums);
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, st, s0, ld, n, g, hist,
-                       order);
+    const dim3 nb((unsigned)tt::ceil_div(n, 256));
+    if (out) {
+        hipLaunchKernelGGL(scatter_rays_kernel, nb, dim3(256), 0, st, s0, ld, n, g, hist, order, staging);
+        hipLaunchKernelGGL(unpack_kernel, nb, dim3(256), 0, st, (const double *)staging, n, out, ldo);
+    } else {
+        hipLaunchKernelGGL(scatter_kernel, nb, dim3(256), 0, st, s0, ld, n, g, hist, order);
+    }
     TT_LAUNCH_CHECK("tt_ray_sort launch");
     return TT_OK;
+}
+}  // namespace
+
+extern "C" int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
+                           void *scratch, int32_t *order, void *stream) {
+    return sort_impl(s0, ld, n, grid, brick_shift, scratch, order, nullptr, nullptr, 0, stream);
+}
+
+extern "C" int tt_ray_sort_rays(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
+                                void *scratch, int32_t *order, double *staging, double *out, int64_t ldo,
+                                void *stream) {
+    TT_REQUIRE(staging && out && ((uintptr_t)staging & 15) == 0, "tt_ray_sort_rays: staging must be 16-byte aligned");
+    TT_REQUIRE(ldo >= n, "tt_ray_sort_rays: ldo < n");
+    TT_REQUIRE(out != s0, "tt_ray_sort_rays: out must not alias s0");
+    return sort_impl(s0, ld, n, grid, brick_shift, scratch, order, staging, out, ldo, stream);
 }

@@ -71,6 +71,7 @@ class Pipeline:
         self.s0_raw = torch.empty((6, N), dtype=f64, device=device)
         self.s0 = torch.empty((6, N), dtype=f64, device=device)
         self.order = torch.empty(N, dtype=torch.int32, device=device)
+        self.staging = torch.empty((N, 8), dtype=f64, device=device)  # one 64-byte record per ray
         nbytes = _lib.C.c_size_t()
         self.lib.tt_ray_sort_scratch_bytes(_lib.C.byref(self.grid), 0, _lib.C.byref(nbytes))
         self.sort_scratch = torch.empty(nbytes.value, dtype=torch.uint8, device=device)
@@ -127,12 +128,9 @@ class Pipeline:
     def sort(self, n=None):
         N = self.cfg.n_rays
         n = N if n is None else n
-        self.lib.tt_ray_sort(_lib.ptr(self.s0_raw), N, n, _lib.C.byref(self.grid), self.cfg.brick_shift,
-                             _lib.ptr(self.sort_scratch), _lib.ptr(self.order), _lib.stream_ptr())
-        if n == N:
-            torch.index_select(self.s0_raw, 1, self.order.long(), out=self.s0)
-        else:
-            self.s0[:, :n].copy_(torch.index_select(self.s0_raw[:, :n], 1, self.order[:n].long()))
+        self.lib.tt_ray_sort_rays(_lib.ptr(self.s0_raw), N, n, _lib.C.byref(self.grid), self.cfg.brick_shift,
+                                  _lib.ptr(self.sort_scratch), _lib.ptr(self.order), _lib.ptr(self.staging),
+                                  _lib.ptr(self.s0), N, _lib.stream_ptr())
 
     def trace(self, n=None):
         N = self.cfg.n_rays
