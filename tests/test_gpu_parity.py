@@ -446,3 +446,53 @@ def test_pipeline_small_end_to_end(lib):
     x = np.linspace(-cfg.L / 2, cfg.L / 2, 64)
     check_tolerance(x, p.rf.cpu().numpy(), p.phase.cpu().numpy(), rf_o, ph_o)
     assert int(p.H_shadow.sum()) > 0.9 * cfg.n_rays
+
+
+# ------------------------------------------------------- probing directions
+def test_probing_y_equals_z_through_swapped_cube(lib):
+    """Probing along y (PT:290-299, 361-368) is probing along z through the
+    cube with its y and z axes swapped: same exit rays, phase and final state
+    bit for bit, on a non-cubic turbulent grid."""
+    from oracle import field as OF
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    x = np.linspace(-5e-3, 5e-3, 33)
+    y = np.linspace(-4e-3, 4e-3, 29)
+    z = np.linspace(-5e-3, 5e-3, 41)
+    d = OF.powerlaw_field((33, 29, 41), -11 / 3, seed=4)
+    ne = 1e24 * (1 + 0.1 * d / np.sqrt((d ** 2).mean()))
+    cy = ElectronCube(x, y, z, probing_direction="y")
+    cy.external_ne(ne)
+    cy.calc_dndr(lwl=1064e-9)
+    s0 = cy.init_beam(3000, 2e-3, 1e-3, seed=5)
+    rf_y = cy.solve(verbose=False).cpu().numpy()
+    cz = ElectronCube(x, z, y, probing_direction="z")
+    cz.external_ne(np.ascontiguousarray(ne.transpose(0, 2, 1)))
+    cz.calc_dndr(lwl=1064e-9)
+    cz.init_beam(1, 2e-3, 1e-3, seed=5)
+    cz.s0 = s0[[0, 2, 1, 3, 5, 4]].contiguous()
+    rf_z = cz.solve(verbose=False).cpu().numpy()
+    np.testing.assert_array_equal(rf_y, rf_z)
+    np.testing.assert_array_equal(cy.phase.cpu().numpy(), cz.phase.cpu().numpy())
+    np.testing.assert_array_equal(cy.sf.cpu().numpy(), cz.sf.cpu().numpy()[[0, 2, 1, 3, 5, 4]])
+    assert np.abs(rf_y[1] - np.arctan(s0[3].cpu().numpy() / s0[4].cpu().numpy())).max() > 1e-7  # deflected
+    # ray_at_exit of the 6-D state gives the same rays
+    np.testing.assert_allclose(cy.ray_at_exit().cpu().numpy(), rf_y, rtol=0, atol=1e-12)
+
+
+def test_probing_x_rays_start_past_the_cube(lib):
+    """The reference launches x-probing rays at x = +extent moving +x
+    (PT:280-289): they never cross the cube, and ray_at_exit (PT:353-360)
+    returns their launch state."""
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    x = np.linspace(-5e-3, 5e-3, 33)
+    cube = ElectronCube(x, x.copy(), x.copy(), probing_direction="x")
+    cube.test_lens()
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(2000, 2e-3, 1e-3, seed=6).cpu().numpy()
+    assert np.all(s0[0] == 5e-3)
+    rf = cube.solve(verbose=False).cpu().numpy()
+    np.testing.assert_allclose(rf[0], s0[1], rtol=0, atol=1e-15)
+    np.testing.assert_allclose(rf[2], s0[2], rtol=0, atol=1e-15)
+    # the ray state carries U, V, W - 1 in float32: angles to float rounding
+    np.testing.assert_allclose(rf[1], np.arctan(s0[4] / s0[3]), rtol=2e-7, atol=0)
+    np.testing.assert_allclose(rf[3], np.arctan(s0[5] / s0[3]), rtol=2e-7, atol=0)

@@ -207,36 +207,78 @@ class ElectronCube:
                                     _lib.ptr(self.s0), Np, _lib.stream_ptr())
         return self.s0
 
-    def _sort_order(self, s0):
+    def _sort_order(self, s0, grid=None):
+        grid = grid or self.grid
         n = s0.shape[1]
         nbytes = _lib.C.c_size_t()
-        self.lib.tt_ray_sort_scratch_bytes(_lib.C.byref(self.grid), 0, _lib.C.byref(nbytes))
+        self.lib.tt_ray_sort_scratch_bytes(_lib.C.byref(grid), 0, _lib.C.byref(nbytes))
         scratch = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
         order = torch.empty(n, dtype=torch.int32, device=self.device)
-        self.lib.tt_ray_sort(_lib.ptr(s0), s0.stride(0), n, _lib.C.byref(self.grid), 0, _lib.ptr(scratch),
+        self.lib.tt_ray_sort(_lib.ptr(s0), s0.stride(0), n, _lib.C.byref(grid), 0, _lib.ptr(scratch),
                              _lib.ptr(order), _lib.stream_ptr())
         return order
 
+    # tracer frame per probing direction: tracer axis a' is cube axis PERM[a'],
+    # so the probing axis becomes the march axis z' and the tracer's exit
+    # rays [x', theta', y', phi'] are the reference's ray_at_exit layout
+    # (PT:353-378): 'x' -> (y, z, x), 'y' -> (x, z, y)
+    PERM = {"x": (1, 2, 0), "y": (0, 2, 1), "z": (0, 1, 2)}
+
+    def _frame(self):
+        """(field4, grid) of the tracer frame of the probing direction; for
+        'x' / 'y' the node field is rebuilt from the axis-permuted density
+        (a strided view, no copy) and kept until calc_dndr runs again."""
+        d = self.probing_direction
+        if d == "z":
+            return self.field4, self.grid
+        cached = getattr(self, "_probe_frame", None)
+        if cached is not None and cached[0] == d and cached[1] is self.field4:
+            return cached[2], cached[3]
+        p = self.PERM[d]
+        g = self.grid
+        n = (g.nx, g.ny, g.nz)
+        o = (g.x0, g.y0, g.z0)
+        sp = (g.dx, g.dy, g.dz)
+        grid = _lib.Grid(n[p[0]], n[p[1]], n[p[2]], o[p[0]], o[p[1]], o[p[2]], sp[p[0]], sp[p[1]], sp[p[2]])
+        ne = self._ne_ref.permute(*p)
+        field4 = torch.empty((grid.nz, grid.ny, grid.nx, 4), dtype=torch.float32, device=self.device)
+        sx, sy, sz = ne.stride()
+        self.lib.tt_grad_stencil(_lib.ptr(ne), 0 if ne.dtype == torch.float32 else 1, sx, sy, sz, _lib.C.byref(grid),
+                                 0.0, 1.0 / self.nc, float(self.ne_max), self.k0, None, 0, _lib.ptr(field4),
+                                 _lib.stream_ptr())
+        self._probe_frame = (d, self.field4, field4, grid)
+        return field4, grid
+
     def solve(self, method="RK45", h=0.5, sort=True, verbose=True, variant=0):
         """Trace all rays through the cube and project them to the exit plane
-        (PT:312-331).  Returns rf (4 x N: x, theta, y, phi)."""
-        if self.probing_direction != "z":
-            raise NotImplementedError("the GPU tracer marches along z; probing_direction must be 'z'")
+        (PT:312-331).  Returns rf (4 x N: the reference's ray_at_exit rows).
+        Probing along 'x' or 'y' traces in an axis-permuted frame whose march
+        axis is the probing axis; as in the reference, 'x' rays start at
+        x = +extent (PT:280-289) and so leave the cube at once."""
         if self.field4 is None:
             raise RuntimeError("call calc_dndr() before solve()")
         extent = self._extent()
-        s0 = _dev(self.s0, device=self.device).contiguous()
+        field4, grid = self._frame()
+        p = self.PERM[self.probing_direction]
+        s0 = _dev(self.s0, device=self.device)
+        if self.probing_direction != "z":
+            s0 = s0[[p[0], p[1], p[2], 3 + p[0], 3 + p[1], 3 + p[2]]]
+        s0 = s0.contiguous()
         n = s0.shape[1]
         self.rf = torch.empty((4, n), dtype=torch.float64, device=self.device)
-        self.sf = torch.empty((6, n), dtype=torch.float64, device=self.device)
+        sf = torch.empty((6, n), dtype=torch.float64, device=self.device)
         self.phase = torch.empty(n, dtype=torch.float64, device=self.device)
         self.status = torch.empty(n, dtype=torch.int32, device=self.device)
         opts = _lib.TraceOpts(float(h), float(np.sqrt(8.0) * extent), int(variant))
         start = time.time()
-        order = self._sort_order(s0) if (sort and n > 0) else None
-        self.lib.tt_trace_rk4(_lib.ptr(self.field4), _lib.C.byref(self.grid), _lib.C.byref(opts), _lib.ptr(s0), n,
-                              n, _lib.ptr(order), _lib.ptr(self.rf), n, _lib.ptr(self.phase), _lib.ptr(self.sf), n,
+        order = self._sort_order(s0, grid) if (sort and n > 0) else None
+        self.lib.tt_trace_rk4(_lib.ptr(field4), _lib.C.byref(grid), _lib.C.byref(opts), _lib.ptr(s0), n, n,
+                              _lib.ptr(order), _lib.ptr(self.rf), n, _lib.ptr(self.phase), _lib.ptr(sf), n,
                               _lib.ptr(self.status), _lib.stream_ptr())
+        if self.probing_direction != "z":  # back to the cube's axes
+            inv = [p.index(a) for a in range(3)]
+            sf = sf[[inv[0], inv[1], inv[2], 3 + inv[0], 3 + inv[1], 3 + inv[2]]]
+        self.sf = sf
         torch.cuda.current_stream().synchronize()
         if verbose:
             print("Ray trace completed in:\t", time.time() - start, "s")
