@@ -80,6 +80,22 @@ int tt_field_generate(tt_fft_plan plan, const tt_spectrum *spec, void *workspace
 int tt_field_from_noise(tt_fft_plan plan, const float *wr, const float *wi, const float *sqrt_psd,
                         void *workspace, float *out, double *sumsq, void *stream);
 
+/* Radial power spectrum of a real field, replacing spectrum_3D_scalar
+ * (gaussian_fields/calculate_spectrum_3d.py:3-59) and compute3Dspectrum
+ * (gaussian_fields/cmpspec.py:147-194).  field: float32 C-order array of
+ * the reference's shape (M0, M1, M2), read with the plan created as
+ * tt_fft_plan_create(M2, M1, M0); workspace as for tt_field_generate.  The
+ * bins are ADDED to sums / counts (float64, nbins each, caller-zeroed).
+ * mode 0 (spectrum_3D_scalar): |fftn(f)|^2 summed, and points counted, in
+ *   bins b <= nbins - 2 of K = sqrt((k0^2 + k1^2) + k2^2), k_i =
+ *   fftfreq(M_i, dx): bin_w b <= K < bin_w (b + 1), bin_w = K.max() / nbins
+ *   computed by the caller as the reference does (C3:45);
+ * mode 1 (compute3Dspectrum): |fftn(f) / nt|^2 summed into shells
+ *   round(sqrt(v0^2 + v1^2 + v2^2)) < nbins over the reference's loop
+ *   values v_i in range(-M_i // 2, M_i // 2 - 1) (CS:175-180); bin_w unused. */
+int tt_power_spectrum(tt_fft_plan plan, const float *field, int32_t mode, int32_t nbins, double dx, double bin_w,
+                      void *workspace, double *sums, double *counts, void *stream);
+
 /* ---------------------------------------------------------------- gradient
  * Replaces calc_dndr (particle_tracking/particle_tracker.py:220-241):
  * ne/nc = alpha + beta * f, clipped at ne_max, then np.gradient semantics

@@ -88,6 +88,7 @@ _SIGNATURES = {
     "tt_fft_workspace_bytes": [P, SIZE_P],
     "tt_field_generate": [P, C.POINTER(Spectrum), P, P, P, P],
     "tt_field_from_noise": [P, P, P, P, P, P, P, P],
+    "tt_power_spectrum": [P, P, I32, I32, D, D, P, P, P, P],
     "tt_grad_stencil": [P, I32, I64, I64, I64, C.POINTER(Grid), D, D, D, D, P, I64, P, P],
     "tt_trace_rk4": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, I64, P, P, I64, P, P],
     "tt_sample_field": [P, C.POINTER(Grid), P, I64, I64, P, I64, P],

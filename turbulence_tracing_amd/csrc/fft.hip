@@ -16,6 +16,7 @@
 // with the twiddle table in LDS.  The spectrum F is never written: pass Z
 // writes the z-transformed half spectrum.  Traffic per field:
 // 8 (Z) + 16 (Y) + 8 + 4 (X) bytes per half-spectrum element.
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -27,6 +28,7 @@ constexpr int kThreads = 256;
 constexpr int kElems = 32;                     // complex values per thread per pass
 constexpr int kBatch = kThreads * kElems;      // complex values per block
 constexpr int kMaxRad = 24;
+constexpr int kMaxBins = 2048;  // power-spectrum bins (two doubles each in LDS)
 
 struct AxisPlan {
     int n, nrad;
@@ -213,7 +215,7 @@ __device__ void lds_fft(float2 *buf, int stride, int B, const AxisPlan &ap, cons
     }
 }
 
-enum { kGenNative = 0, kGenNoise = 1, kPassY = 2, kPassX = 3 };
+enum { kGenNative = 0, kGenNoise = 1, kPassY = 2, kPassX = 3, kFwdX = 4, kPowZ = 5 };
 
 struct PassArgs {
     AxisPlan ap;
@@ -231,6 +233,13 @@ struct PassArgs {
     float *out;
     float scale;
     double *sumsq;
+    // power spectrum (kFwdX input, kPowZ binning)
+    const float *in;
+    int mode, nbins;
+    double fz_step, fy_step, fx_step;  // 1 / (n dx) per axis (spectrum_3D_scalar frequencies)
+    double bin_w;                      // bin width in |k| (spectrum_3D_scalar)
+    double pscale;                     // factor on |F|^2 (compute3Dspectrum: 1 / nt^2)
+    double *hsum, *hcnt;
 };
 
 __device__ __forceinline__ int centred(int j, int n) { return (j + n / 2) % n; }
@@ -283,6 +292,63 @@ __device__ __forceinline__ float2 gen(const PassArgs &a, int jx, int jy, int jz)
     return make_float2((wc.x + wp.x) * s, (wc.y - wp.y) * s);
 }
 
+// ---------------------------------------------------------------------
+// Power spectrum of a real field (the estimators of
+// gaussian_fields/calculate_spectrum_3d.py:3-59 and cmpspec.py:147-194).
+// The passes above run with the inverse sign; for a real field that gives
+// conj(fftn(f)), whose |.|^2 is fftn's.  The field is read as rows along x'
+// (the reference's last axis, contiguous in C order), so x' / y' / z' are
+// the reference's axes 2 / 1 / 0.  Each stored half-spectrum element also
+// stands for its mirror (-k), which has the same power, unless kx' is 0 or
+// nx'/2.
+// Mode 0 (spectrum_3D_scalar): K = sqrt((k0^2 + k1^2) + k2^2) with k_i the
+//   fftfreq(M_i, dx) value of axis i; bin b holds bin_w (b) <= K <
+//   bin_w (b + 1), the reference's comparisons, for b <= nbins - 2.
+// Mode 1 (compute3Dspectrum): shell round(sqrt(v0^2 + v1^2 + v2^2)) of the
+//   reference's loop values v_i in range(-M_i // 2, M_i // 2 - 1), so one
+//   index per axis, (M_i - 2) // 2, is never visited; power scaled by
+//   1 / nt^2 (rh = fftn(r) / nt).
+__device__ __forceinline__ int signed_of(int j, int n) { return (j <= (n - 1) / 2) ? j : j - n; }
+// the reference loop value of array index j, or INT_MIN when not visited
+__device__ __forceinline__ int loop_value(int j, int n) {
+    const int hi = n / 2 - 2;
+    if (j == (n - 2) / 2) return INT_MIN;
+    return j <= hi ? j : j - n;
+}
+
+#pragma clang fp contract(off)
+__device__ __forceinline__ void bin_power(const PassArgs &a, int jx, int jy, int jz, float power, double *hs,
+                                          double *hc) {
+    const bool self_conj = jx == 0 || (a.nx % 2 == 0 && jx == a.nx / 2);
+    if (a.mode == 0) {
+        const double f0 = (double)signed_of(jz, a.nz) * a.fz_step;
+        const double f1 = (double)signed_of(jy, a.ny) * a.fy_step;
+        const double f2 = (double)signed_of(jx, a.nx) * a.fx_step;
+        const double K = sqrt((f0 * f0 + f1 * f1) + f2 * f2);
+        int b = (int)(K / a.bin_w);
+        if (K < (double)b * a.bin_w) --b;
+        else if (K >= (double)(b + 1) * a.bin_w) ++b;
+        if (b < 0 || b > a.nbins - 2) return;
+        const double w = self_conj ? 1.0 : 2.0;
+        atomicAdd(&hs[b], w * (double)power);
+        atomicAdd(&hc[b], w);
+        return;
+    }
+    const double p = (double)power * a.pscale;
+    for (int m = 0; m < (self_conj ? 1 : 2); ++m) {
+        const int ix = m ? (a.nx - jx) % a.nx : jx, iy = m ? (a.ny - jy) % a.ny : jy, iz = m ? (a.nz - jz) % a.nz : jz;
+        const int v0 = loop_value(iz, a.nz), v1 = loop_value(iy, a.ny), v2 = loop_value(ix, a.nx);
+        if (v0 == INT_MIN || v1 == INT_MIN || v2 == INT_MIN) continue;
+        const double rk = sqrt((double)v0 * v0 + (double)v1 * v1 + (double)v2 * v2);
+        const int sh = (int)rint(rk);
+        if (sh < a.nbins) {
+            atomicAdd(&hs[sh], p);
+            atomicAdd(&hc[sh], 1.0);
+        }
+    }
+}
+#pragma clang fp contract(on)
+
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
     extern __shared__ float2 smem[];
@@ -294,7 +360,51 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
     const int B = a.B;
     const int total = B * n;
 
-    if (MODE == kGenNative || MODE == kGenNoise || MODE == kPassY) {
+    if (MODE == kPowZ) {
+        // pencils along z of the half spectrum, then |F|^2 into a block-private
+        // histogram (after the FFT's LDS) flushed with global atomics
+        double *hs = reinterpret_cast<double *>(tw + n), *hc = hs + a.nbins;
+        for (int j = threadIdx.x; j < 2 * a.nbins; j += kThreads) hs[j] = 0.0;
+        const int kx0 = blockIdx.x * B;
+        const int fixed = blockIdx.y;
+        for (int g = threadIdx.x; g < total; g += kThreads) {
+            const int p = g % B, j = g / B;
+            const int kx = kx0 + p;
+            buf[p * stride + j] = kx < a.nxh ? a.spec[((int64_t)j * a.ny + fixed) * a.nxh + kx] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        lds_fft(buf, stride, B, a.ap, tw);
+        for (int g = threadIdx.x; g < total; g += kThreads) {
+            const int p = g % B, j = g / B;
+            const int kx = kx0 + p;
+            if (kx >= a.nxh) continue;
+            const float2 v = buf[p * stride + j];
+            bin_power(a, kx, fixed, j, v.x * v.x + v.y * v.y, hs, hc);
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < a.nbins; j += kThreads) {
+            if (hc[j] != 0.0) {
+                atomicAdd(&a.hsum[j], hs[j]);
+                atomicAdd(&a.hcnt[j], hc[j]);
+            }
+        }
+    } else if (MODE == kFwdX) {
+        // real rows along x' -> first nx'/2 + 1 outputs of their transform
+        const int64_t rows = (int64_t)a.ny * a.nz;
+        const int64_t r0 = (int64_t)blockIdx.x * B;
+        for (int g = threadIdx.x; g < total; g += kThreads) {
+            const int p = g / n, k = g - p * n;
+            const int64_t row = r0 + p;
+            buf[p * stride + k] = make_float2(row < rows ? a.in[row * n + k] : 0.f, 0.f);
+        }
+        __syncthreads();
+        lds_fft(buf, stride, B, a.ap, tw);
+        for (int g = threadIdx.x; g < total; g += kThreads) {
+            const int p = g / n, k = g - p * n;
+            const int64_t row = r0 + p;
+            if (row < rows && k < a.nxh) a.spec[row * a.nxh + k] = buf[p * stride + k];
+        }
+    } else if (MODE == kGenNative || MODE == kGenNoise || MODE == kPassY) {
         // pencils along z (gen) or y: B consecutive kx columns at fixed ky (gen) / z (Y)
         const int kx0 = blockIdx.x * B;
         const int fixed = blockIdx.y;
@@ -466,9 +576,12 @@ int enable_lds() {
     static int rc = -1;
     if (rc >= 0) return rc;
     // the largest request: n = 4096 (2 pencils per block), rounded up to 1 KB
-    const int want = (int)((smem_bytes(4096, pencils_per_block(4096)) + 1023) / 1024 * 1024);
-    const void *ks[4] = {(const void *)fft_pass_kernel<kGenNative>, (const void *)fft_pass_kernel<kGenNoise>,
-                         (const void *)fft_pass_kernel<kPassY>, (const void *)fft_pass_kernel<kPassX>};
+    // (98 KB + 32 KB of power-spectrum bins, within the 160 KB of a CU)
+    const int want = (int)((smem_bytes(4096, pencils_per_block(4096)) + 2 * kMaxBins * sizeof(double) + 1023) /
+                           1024 * 1024);
+    const void *ks[6] = {(const void *)fft_pass_kernel<kGenNative>, (const void *)fft_pass_kernel<kGenNoise>,
+                         (const void *)fft_pass_kernel<kPassY>,     (const void *)fft_pass_kernel<kPassX>,
+                         (const void *)fft_pass_kernel<kFwdX>,      (const void *)fft_pass_kernel<kPowZ>};
     for (const void *k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, want);
         if (e != hipSuccess) return rc = tt::hip_check(e, "hipFuncSetAttribute(max dynamic LDS)");
@@ -547,4 +660,56 @@ extern "C" int tt_field_from_noise(tt_fft_plan plan, const float *wr, const floa
     a.wi = wi;
     a.sp = sqrt_psd;
     return run_passes(plan, a, false, workspace, out, sumsq, tt::as_stream(stream));
+}
+
+extern "C" int tt_power_spectrum(tt_fft_plan plan, const float *field, int32_t mode, int32_t nbins, double dx,
+                                 double bin_w, void *workspace, double *sums, double *counts, void *stream) {
+    TT_REQUIRE(plan && field && workspace && sums && counts, "tt_power_spectrum: null pointer");
+    TT_REQUIRE(mode == 0 || mode == 1, "tt_power_spectrum: mode must be 0 (spectrum_3D_scalar) or 1 (compute3Dspectrum)");
+    TT_REQUIRE(nbins >= 1 && nbins <= kMaxBins, "tt_power_spectrum: nbins must be in [1, %d]", kMaxBins);
+    TT_REQUIRE(dx > 0, "tt_power_spectrum: dx must be > 0");
+    TT_REQUIRE(mode == 1 || bin_w > 0, "tt_power_spectrum: bin_w must be > 0");
+    if (int rc = enable_lds()) return rc;
+    hipStream_t st = tt::as_stream(stream);
+    PassArgs a{};
+    a.nx = plan->nx;
+    a.ny = plan->ny;
+    a.nz = plan->nz;
+    a.nxh = plan->nxh;
+    a.spec = static_cast<float2 *>(workspace);
+    a.in = field;
+    a.mode = mode;
+    a.nbins = nbins;
+    // numpy fftfreq(M, d): integer index times 1.0 / (M * d)
+    a.fx_step = 1.0 / ((double)plan->nx * dx);
+    a.fy_step = 1.0 / ((double)plan->ny * dx);
+    a.fz_step = 1.0 / ((double)plan->nz * dx);
+    a.bin_w = bin_w;
+    const double nt = (double)plan->nx * plan->ny * plan->nz;
+    a.pscale = 1.0 / (nt * nt);
+    a.hsum = sums;
+    a.hcnt = counts;
+    // pass X: real rows -> half spectrum
+    a.ap = plan->ax[0];
+    a.tw = plan->tw[0];
+    a.B = pencils_per_block(plan->nx);
+    hipLaunchKernelGGL(fft_pass_kernel<kFwdX>, dim3((unsigned)tt::ceil_div((int64_t)plan->ny * plan->nz, a.B)),
+                       dim3(kThreads), smem_bytes(plan->nx, a.B), st, a);
+    TT_LAUNCH_CHECK("power spectrum pass X launch");
+    // pass Y
+    a.ap = plan->ax[1];
+    a.tw = plan->tw[1];
+    a.B = pencils_per_block(plan->ny);
+    hipLaunchKernelGGL(fft_pass_kernel<kPassY>, dim3((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->nz),
+                       dim3(kThreads), smem_bytes(plan->ny, a.B), st, a);
+    TT_LAUNCH_CHECK("power spectrum pass Y launch");
+    // pass Z with |F|^2 binning (sums and counts are ADDED to)
+    a.ap = plan->ax[2];
+    a.tw = plan->tw[2];
+    a.B = pencils_per_block(plan->nz);
+    const size_t sm = smem_bytes(plan->nz, a.B) + 2 * (size_t)nbins * sizeof(double);
+    hipLaunchKernelGGL(fft_pass_kernel<kPowZ>, dim3((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->ny),
+                       dim3(kThreads), sm, st, a);
+    TT_LAUNCH_CHECK("power spectrum pass Z launch");
+    return TT_OK;
 }
