@@ -496,3 +496,25 @@ def test_probing_x_rays_start_past_the_cube(lib):
     # the ray state carries U, V, W - 1 in float32: angles to float rounding
     np.testing.assert_allclose(rf[1], np.arctan(s0[4] / s0[3]), rtol=2e-7, atol=0)
     np.testing.assert_allclose(rf[3], np.arctan(s0[5] / s0[3]), rtol=2e-7, atol=0)
+
+
+def test_detectors_pickle_like_the_reference(lib, tmp_path):
+    """Diagnostics pickle (mpi_example.py:154-166) to numpy-backed objects
+    that load without touching the GPU and keep H, edges and rays."""
+    import pickle
+    from turbulence_tracing_amd import ray_transfer_matrix as rtm
+    rs = np.random.RandomState(3)
+    r0 = np.stack([rs.randn(5000) * 1e-3, rs.randn(5000) * 1e-4, rs.randn(5000) * 1e-3, rs.randn(5000) * 1e-4])
+    sh = rtm.Shadowgraphy(r0, L=400, R=25, Lx=18, Ly=13.5)
+    sh.solve()
+    sh.histogram(bin_scale=10)
+    path = tmp_path / "Shadowgraphy.pkl"
+    with open(path, "wb") as f:
+        pickle.dump(sh, f)
+    with open(path, "rb") as f:
+        back = pickle.load(f)  # written by this test
+    assert isinstance(back.H, np.ndarray) and isinstance(back.rf, np.ndarray)
+    np.testing.assert_array_equal(back.H, sh.H.cpu().numpy())
+    np.testing.assert_array_equal(back.xedges, sh.xedges)
+    assert back.H.sum() == np.isfinite(sh.rf.cpu().numpy()).all(0).sum() - (
+        (np.abs(sh.rf.cpu().numpy()[0]) > 9) | (np.abs(sh.rf.cpu().numpy()[2]) > 6.75)).sum()

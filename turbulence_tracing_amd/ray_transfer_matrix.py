@@ -191,6 +191,19 @@ class Rays:
         self.r0 = None
         self.rf = None
 
+    # The reference's drivers pickle whole diagnostics (mpi_example.py:154-166).
+    # Device tensors are stored as numpy arrays, so a pickle loads on a host
+    # without a GPU and reads like the reference's (H, rays as numpy).
+    def __getstate__(self):
+        st = dict(self.__dict__)
+        for k, v in st.items():
+            if isinstance(v, torch.Tensor):
+                st[k] = v.detach().cpu().numpy()
+        return st
+
+    def __setstate__(self, st):
+        self.__dict__.update(st)
+
 
 def shadowgraphy_ops(L, R, focal_plane):
     """RTM:214-227."""
