@@ -262,11 +262,12 @@ __global__ void standardize_kernel(const float *x, int64_t srow, int64_t scol, i
 // nt * KC + c, each 64 lanes x 8 bf16 (lane l: W[16nt + l%16][kperm(c, l/16,
 // j)]), zero-padded.  Layer shapes (NT x KC): 16 x 2, 16 x 8, 16 x 8, 1 x 8.
 // A block of 4 waves takes 128 rays (32 per wave, two 16-ray B tiles) and
-// stages the fragments of up to 8 output tiles at a time through 64 KB of
-// LDS, so each fragment is read once from L2 per block and once from LDS per
-// wave for two MFMAs.
+// stages the fragments of up to 4 output tiles at a time through 32 KB of
+// LDS (146 VGPRs: three blocks per CU), so each fragment is read once from L2
+// per block and once from LDS per wave for two MFMAs.  The accumulators start
+// from the bias, so the epilogue is GELU and the bf16 conversion only.
 constexpr int kInferRays = 128;
-constexpr int kLdsFrags = 64;  // 64 KB
+constexpr int kLdsFrags = 32;  // 32 KB
 
 struct InferArgs {
     const float *feat;    // element (ray r, feature f) at feat[r * srow + f * scol]
@@ -300,20 +301,20 @@ __device__ __forceinline__ void hidden_layer(const __bf16 *w, const float *bias,
         __syncthreads();
 #pragma unroll
         for (int tl = 0; tl < NG; ++tl) {
-            f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+            // the accumulators start from the bias (lane l: features 4(l/16) + i)
+            const int nt = grp * NG + tl;
+            const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * nt + 4 * g);
+            f32x4 acc0 = {bb.x, bb.y, bb.z, bb.w}, acc1 = acc0;
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
                 const bf16x8 a = *reinterpret_cast<const bf16x8 *>(lds + ((tl * KC + c) * 64 + lane) * 8);
                 acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][0], acc0, 0, 0, 0);
                 acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][1], acc1, 0, 0, 0);
             }
-            const int nt = grp * NG + tl;
-            const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * nt + 4 * g);
-            const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                y[nt >> 1][0][4 * (nt & 1) + i] = (__bf16)gelu_f(acc0[i] + bv[i]);
-                y[nt >> 1][1][4 * (nt & 1) + i] = (__bf16)gelu_f(acc1[i] + bv[i]);
+                y[nt >> 1][0][4 * (nt & 1) + i] = (__bf16)gelu_f(acc0[i]);
+                y[nt >> 1][1][4 * (nt & 1) + i] = (__bf16)gelu_f(acc1[i]);
             }
         }
     }
