@@ -12,7 +12,6 @@ reference function: the reference has no surrogate.  Pins:
     float64 on the CPU.
 """
 import numpy as np
-from scipy.special import erf
 
 from .tracer import C, trilinear
 

@@ -4,7 +4,6 @@ bin counts exactly, binned power to float32 FFT accuracy, on even, odd and
 mixed grid sizes."""
 import numpy as np
 import pytest
-import torch
 
 from oracle import spectrum as OS
 

@@ -11,7 +11,6 @@ def test_fused_inference_packing_emulated():
     (lane l: A[l%16][8(l/16)+j], B[8(l/16)+j][l%16], D[4(l/16)+i][l%16]) with
     the packed weights, each layer's D tiles reused as the next layer's B
     operand in place.  Equals the plain forward pass."""
-    import torch
     from turbulence_tracing_amd import surrogate as S
     rs = np.random.RandomState(0)
     dims = (48, 256, 256, 256, 8)

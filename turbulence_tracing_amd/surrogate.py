@@ -1,22 +1,23 @@
 """Learned correction of a cheap path-integrated ray estimate, with the full
-RK4 tracer as fallback and judge.
+RK4 tracer as the fallback and the measure of its error.
 
 Not part of the reference (which has no surrogate).  The pieces:
-  * ``tt_path_features`` integrates the deflection and phase along each ray's
-    path with one field evaluation per z-cell (a drift-kick-drift step; the
-    full tracer takes eight) and returns the estimate of the exit state
-    [x, theta, y, phi, phase] plus 48 features;
+  * ``tt_field_blocks`` / ``tt_path_features`` integrate the field along
+    straight lines over z-blocks of k cells (with first-moment corrections)
+    and advance each ray one drift-kick-drift step per block (the full
+    tracer takes eight field evaluations per cell); the result is the
+    estimate of the exit state [x, theta, y, phi, phase] plus 48 features;
   * a 4-layer MLP (48 -> 256 -> 256 -> 256 -> 8, GELU, bf16 weights and
-    activations, fp32 accumulation; hand-written MFMA GEMMs in libtt)
-    predicts the residual (full trace - estimate) for x, y, theta, phi,
-    phase and two error bounds (position, angle);
+    activations, fp32 accumulation; hand-written MFMA kernels in libtt for
+    training and one fused kernel for inference) predicts the residual
+    (full trace - estimate) for x, y, theta, phi, phase and two error
+    bounds (position, angle);
   * rays whose predicted error bound exceeds the threshold (1 um / 1 urad by
     default), or whose estimate is not finite, are re-traced with
     ``tt_trace_rk4``.
 Training data are rays the tracer itself generates through seeded fields.
 """
 import ctypes as C
-import time
 from dataclasses import dataclass, field
 
 import numpy as np

@@ -10,9 +10,6 @@ histograms.  The reference does the same work in one solve over all rays
 (particle_tracking/particle_tracker.py:312-331) or split across processes
 whose histograms are summed (particle_tracking/mpi_example.py:136-149).
 """
-import numpy as np
-import torch
-
 from .pipeline import Config, Pipeline
 
 
