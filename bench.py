@@ -171,6 +171,7 @@ def main():
                               "calibrated on a 1 GiB copy)",
             "trace_ms": trace_ms,
             "bytes_per_ray_step": BYTES_PER_RAY_STEP,
+            "compute": profiled_valu(cfg),
         },
         "cpu_baseline": None,
     }
@@ -314,6 +315,27 @@ def profiled_traffic(cfg):
     except OSError:
         pass
     return None
+
+
+def profiled_valu(cfg):
+    """The trace kernel's VALU issue, measured by rocprofv3 (SQ_INSTS_VALU
+    over the kernel's cycles) for the 512^3 / 1e7-ray workload: the kernel
+    is bound by instruction issue, not by HBM (its measured traffic is a
+    few GB per launch)."""
+    if (cfg.n, cfg.n_rays, cfg.variant) != (512, 10_000_000, 0):
+        return None
+    path = os.path.join(ROOT, "profiles", "trace_rk4_512_1e7_valu.md")
+    out = {}
+    try:
+        for line in open(path):
+            if line.startswith("trace_rk4_valu_wave_instr_per_step:"):
+                out["valu_wave_instr_per_step"] = int(line.split(":")[1])
+            elif line.startswith("trace_rk4_valu_frac:"):
+                out["valu_frac_of_issue_peak"] = float(line.split(":")[1])
+    except OSError:
+        return None
+    out["source"] = "profiles/trace_rk4_512_1e7_valu.md"
+    return out or None
 
 
 def surrogate_leg(p, cfg, n_fields, k, stages):
