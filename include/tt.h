@@ -160,6 +160,7 @@ typedef struct {
 
 typedef struct {
     int32_t nitems, nchunks, chunk;
+    int32_t steps_per_cell; /* m = 1/h */
 } tt_batch_info;
 
 typedef struct {

@@ -70,7 +70,8 @@ class BatchItem(C.Structure):
 
 
 class BatchInfo(C.Structure):
-    _fields_ = [("nitems", C.c_int32), ("nchunks", C.c_int32), ("chunk", C.c_int32)]
+    _fields_ = [("nitems", C.c_int32), ("nchunks", C.c_int32), ("chunk", C.c_int32),
+                ("steps_per_cell", C.c_int32)]
 
 
 class BatchTuning(C.Structure):

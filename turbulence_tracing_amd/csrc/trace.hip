@@ -48,8 +48,11 @@ struct TraceArgs {
     int32_t *status;
 };
 
+// qx, qy: (x, y) displacement from the entry point in units of dz (the
+// displacement in cells is (dz/dx) qx, (dz/dy) qy); Wd = W - 1;
+// tr = int (1/W - 1) dZ
 struct State {
-    float dpx, dpy, U, V, Wd, tr;  // Wd = W - 1, tr = int (1/W - 1) dZ
+    float qx, qy, U, V, Wd, tr;
 };
 
 __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
@@ -74,42 +77,54 @@ __device__ __forceinline__ float4 sample(const TraceArgs &a, float px, float py,
     return lerp4(lerp4(c00, c10, ty), lerp4(c01, c11, ty), tz);
 }
 
+// Path-length factor ds/(dz dZ) = |v| / W of a state.  The phase slope of a
+// stage is Phi |v| / W; the RK4 steps take the field part Phi at the four
+// stage points and |v| / W once, at the third stage's state (the step's
+// midpoint): |v| / W = sqrt(1 + tan^2) changes by O(angle x deflection) per
+// cell, so the midpoint value differs from per-stage values by far less
+// than the fp32 rounding of the increment.
+__device__ __forceinline__ float path_factor(const State &s, float W, float iW) {
+    return __builtin_amdgcn_sqrtf(fmaf(s.U, s.U, fmaf(s.V, s.V, W * W))) * iW;
+}
+
+template <bool VF>
 __device__ __forceinline__ void rhs(const TraceArgs &a, float pxb, float pyb, float Z, const State &s, State &k,
-                                    float &kph) {
-    const float4 g = sample(a, pxb + s.dpx, pyb + s.dpy, Z);
+                                    float &gw, float &vf) {
+    const float4 g = sample(a, fmaf(a.sx, s.qx, pxb), fmaf(a.sy, s.qy, pyb), Z);
     const float W = 1.0f + s.Wd;
     const float iW = 1.0f / W;
-    k.dpx = a.sx * s.U * iW;
-    k.dpy = a.sy * s.V * iW;
+    k.qx = s.U * iW;
+    k.qy = s.V * iW;
     k.U = g.x * iW;
     k.V = g.y * iW;
     k.Wd = g.z * iW;
     k.tr = -s.Wd * iW;
-    kph = g.w * sqrtf(fmaf(s.U, s.U, fmaf(s.V, s.V, W * W))) * iW;
+    gw = g.w;
+    if (VF) vf = path_factor(s, W, iW);
 }
 
 __device__ __forceinline__ State axpy(const State &y, float c, const State &k) {
-    return State{fmaf(c, k.dpx, y.dpx), fmaf(c, k.dpy, y.dpy), fmaf(c, k.U, y.U),
-                 fmaf(c, k.V, y.V),     fmaf(c, k.Wd, y.Wd),   fmaf(c, k.tr, y.tr)};
+    return State{fmaf(c, k.qx, y.qx), fmaf(c, k.qy, y.qy), fmaf(c, k.U, y.U),
+                 fmaf(c, k.V, y.V),   fmaf(c, k.Wd, y.Wd), fmaf(c, k.tr, y.tr)};
 }
 
 // one classic RK4 step of size hh from Z; returns the phase increment
 __device__ __forceinline__ float rk4_step(const TraceArgs &a, float pxb, float pyb, float Z, float hh, State &y) {
     State k1, k2, k3, k4;
-    float p1, p2, p3, p4;
+    float p1, p2, p3, p4, vf = 0.f;
     const float h2 = 0.5f * hh;
-    rhs(a, pxb, pyb, Z, y, k1, p1);
-    rhs(a, pxb, pyb, Z + h2, axpy(y, h2, k1), k2, p2);
-    rhs(a, pxb, pyb, Z + h2, axpy(y, h2, k2), k3, p3);
-    rhs(a, pxb, pyb, Z + hh, axpy(y, hh, k3), k4, p4);
+    rhs<false>(a, pxb, pyb, Z, y, k1, p1, vf);
+    rhs<false>(a, pxb, pyb, Z + h2, axpy(y, h2, k1), k2, p2, vf);
+    rhs<true>(a, pxb, pyb, Z + h2, axpy(y, h2, k2), k3, p3, vf);
+    rhs<false>(a, pxb, pyb, Z + hh, axpy(y, hh, k3), k4, p4, vf);
     const float h6 = hh * (1.0f / 6.0f);
-    y.dpx += h6 * (k1.dpx + 2.f * (k2.dpx + k3.dpx) + k4.dpx);
-    y.dpy += h6 * (k1.dpy + 2.f * (k2.dpy + k3.dpy) + k4.dpy);
+    y.qx += h6 * (k1.qx + 2.f * (k2.qx + k3.qx) + k4.qx);
+    y.qy += h6 * (k1.qy + 2.f * (k2.qy + k3.qy) + k4.qy);
     y.U += h6 * (k1.U + 2.f * (k2.U + k3.U) + k4.U);
     y.V += h6 * (k1.V + 2.f * (k2.V + k3.V) + k4.V);
     y.Wd += h6 * (k1.Wd + 2.f * (k2.Wd + k3.Wd) + k4.Wd);
     y.tr += h6 * (k1.tr + 2.f * (k2.tr + k3.tr) + k4.tr);
-    return h6 * (p1 + 2.f * (p2 + p3) + p4);
+    return h6 * (p1 + 2.f * (p2 + p3) + p4) * vf;
 }
 
 // ---------------------------------------------------------------------------
@@ -155,21 +170,22 @@ __device__ __forceinline__ float4 eval_cell(const Cell &C, float tx, float ty, f
 }
 
 // one RK4 stage from the cached cell; ok &= the stage point lies in the cell
-template <bool TAU>
+template <bool TAU, bool VF>
 __device__ __forceinline__ void rhs_c(const TraceArgs &a, const Cell &C, float ox, float oy, float tz, const State &s,
-                                      State &k, float &kph, bool &ok) {
-    const float tx = ox + s.dpx, ty = oy + s.dpy;  // ox = pxb - ix
+                                      State &k, float &gw, float &vf, bool &ok) {
+    const float tx = fmaf(a.sx, s.qx, ox), ty = fmaf(a.sy, s.qy, oy);  // ox = pxb - ix
     ok = ok && tx >= 0.f && tx <= 1.f && ty >= 0.f && ty <= 1.f;
     const float4 g = eval_cell(C, tx, ty, tz);
     const float W = 1.0f + s.Wd;
     const float iW = __builtin_amdgcn_rcpf(W);
-    k.dpx = a.sx * s.U * iW;
-    k.dpy = a.sy * s.V * iW;
+    k.qx = s.U * iW;
+    k.qy = s.V * iW;
     k.U = g.x * iW;
     k.V = g.y * iW;
     k.Wd = g.z * iW;
     k.tr = TAU ? -s.Wd * iW : 0.f;
-    kph = g.w * __builtin_amdgcn_sqrtf(fmaf(s.U, s.U, fmaf(s.V, s.V, W * W))) * iW;
+    gw = g.w;
+    if (VF) vf = path_factor(s, W, iW);
 }
 
 template <bool TAU>
@@ -177,28 +193,28 @@ __device__ __forceinline__ float rk4_step_c(const TraceArgs &a, const Cell &C, f
                                             State &y, bool &ok) {
     // running sum acc = k1 + 2 k2 + 2 k3 + k4 keeps one stage's slopes live
     State k, acc;
-    float p, pacc;
+    float p, pacc, vf = 0.f;
     const float h2 = 0.5f * hh;
-    rhs_c<TAU>(a, C, ox, oy, tz0, y, k, p, ok);
+    rhs_c<TAU, false>(a, C, ox, oy, tz0, y, k, p, vf, ok);
     acc = k;
     pacc = p;
-    rhs_c<TAU>(a, C, ox, oy, tz0 + h2, axpy(y, h2, k), k, p, ok);
+    rhs_c<TAU, false>(a, C, ox, oy, tz0 + h2, axpy(y, h2, k), k, p, vf, ok);
     acc = axpy(acc, 2.f, k);
     pacc = fmaf(2.f, p, pacc);
-    rhs_c<TAU>(a, C, ox, oy, tz0 + h2, axpy(y, h2, k), k, p, ok);
+    rhs_c<TAU, true>(a, C, ox, oy, tz0 + h2, axpy(y, h2, k), k, p, vf, ok);
     acc = axpy(acc, 2.f, k);
     pacc = fmaf(2.f, p, pacc);
-    rhs_c<TAU>(a, C, ox, oy, tz0 + hh, axpy(y, hh, k), k, p, ok);
+    rhs_c<TAU, false>(a, C, ox, oy, tz0 + hh, axpy(y, hh, k), k, p, vf, ok);
     acc = axpy(acc, 1.f, k);
     pacc += p;
     const float h6 = hh * (1.0f / 6.0f);
-    y.dpx = fmaf(h6, acc.dpx, y.dpx);
-    y.dpy = fmaf(h6, acc.dpy, y.dpy);
+    y.qx = fmaf(h6, acc.qx, y.qx);
+    y.qy = fmaf(h6, acc.qy, y.qy);
     y.U = fmaf(h6, acc.U, y.U);
     y.V = fmaf(h6, acc.V, y.V);
     y.Wd = fmaf(h6, acc.Wd, y.Wd);
     if (TAU) y.tr = fmaf(h6, acc.tr, y.tr);
-    return h6 * pacc;
+    return h6 * pacc * vf;
 }
 
 // ---------------------------------------------------------------------------
@@ -214,26 +230,23 @@ struct ZP {
     float4 A, B, Cc, E;  // f = A + B tx + Cc ty + E tx ty on one z-plane
 };
 
-__device__ __forceinline__ ZP zplane(const Cell &C, float tz) {
-    return ZP{f4fma(C.d, tz, C.a), f4fma(C.f, tz, C.b), f4fma(C.g, tz, C.c), f4fma(C.h, tz, C.e)};
-}
-
-template <bool TAU>
+template <bool TAU, bool VF>
 __device__ __forceinline__ void rhs_z(const TraceArgs &a, const ZP &z, float ox, float oy, const State &s, State &k,
-                                      float &kph, float &mn, float &mx) {
-    const float tx = ox + s.dpx, ty = oy + s.dpy;
+                                      float &gw, float &vf, float &mn, float &mx) {
+    const float tx = fmaf(a.sx, s.qx, ox), ty = fmaf(a.sy, s.qy, oy);
     mn = fminf(mn, fminf(tx, ty));
     mx = fmaxf(mx, fmaxf(tx, ty));
     const float4 g = f4fma(f4fma(z.E, ty, z.B), tx, f4fma(z.Cc, ty, z.A));
     const float W = 1.0f + s.Wd;
     const float iW = __builtin_amdgcn_rcpf(W);
-    k.dpx = a.sx * s.U * iW;
-    k.dpy = a.sy * s.V * iW;
+    k.qx = s.U * iW;
+    k.qy = s.V * iW;
     k.U = g.x * iW;
     k.V = g.y * iW;
     k.Wd = g.z * iW;
     k.tr = TAU ? -s.Wd * iW : 0.f;
-    kph = g.w * __builtin_amdgcn_sqrtf(fmaf(s.U, s.U, fmaf(s.V, s.V, W * W))) * iW;
+    gw = g.w;
+    if (VF) vf = path_factor(s, W, iW);
 }
 
 // one RK4 step of h = 1/2 over the planes z0 (start), zm (middle), z1 (end)
@@ -241,27 +254,116 @@ template <bool TAU>
 __device__ __forceinline__ float rk4_half_z(const TraceArgs &a, const ZP &z0, const ZP &zm, const ZP &z1, float ox,
                                             float oy, State &y, float &mn, float &mx) {
     State k, acc;
-    float p, pacc;
-    rhs_z<TAU>(a, z0, ox, oy, y, k, p, mn, mx);
+    float p, pacc, vf = 0.f;
+    rhs_z<TAU, false>(a, z0, ox, oy, y, k, p, vf, mn, mx);
     acc = k;
     pacc = p;
-    rhs_z<TAU>(a, zm, ox, oy, axpy(y, 0.25f, k), k, p, mn, mx);
+    rhs_z<TAU, false>(a, zm, ox, oy, axpy(y, 0.25f, k), k, p, vf, mn, mx);
     acc = axpy(acc, 2.f, k);
     pacc = fmaf(2.f, p, pacc);
-    rhs_z<TAU>(a, zm, ox, oy, axpy(y, 0.25f, k), k, p, mn, mx);
+    rhs_z<TAU, true>(a, zm, ox, oy, axpy(y, 0.25f, k), k, p, vf, mn, mx);
     acc = axpy(acc, 2.f, k);
     pacc = fmaf(2.f, p, pacc);
-    rhs_z<TAU>(a, z1, ox, oy, axpy(y, 0.5f, k), k, p, mn, mx);
+    rhs_z<TAU, false>(a, z1, ox, oy, axpy(y, 0.5f, k), k, p, vf, mn, mx);
     acc = axpy(acc, 1.f, k);
     pacc += p;
     const float h6 = 0.5f * (1.0f / 6.0f);
-    y.dpx = fmaf(h6, acc.dpx, y.dpx);
-    y.dpy = fmaf(h6, acc.dpy, y.dpy);
+    y.qx = fmaf(h6, acc.qx, y.qx);
+    y.qy = fmaf(h6, acc.qy, y.qy);
     y.U = fmaf(h6, acc.U, y.U);
     y.V = fmaf(h6, acc.V, y.V);
     y.Wd = fmaf(h6, acc.Wd, y.Wd);
     if (TAU) y.tr = fmaf(h6, acc.tr, y.tr);
-    return h6 * pacc;
+    return h6 * pacc * vf;
+}
+
+// The cell as plane forms: z0 on its lower z-plane and D = (upper - lower),
+// so the plane at tz is z0 + tz D (same coefficients as load_cell: A, B, Cc,
+// E = a, b, c, e and D = d, f, g, h).
+__device__ __forceinline__ void load_planes(const TraceArgs &A, int ix, int iy, int iz, ZP &z0, ZP &D) {
+    const int64_t plane = (int64_t)A.nx * A.ny;
+    const float4 *p = A.field + ((int64_t)iz * A.ny + iy) * A.nx + ix;
+    const float4 c000 = p[0], c100 = p[1], c010 = p[A.nx], c110 = p[A.nx + 1];
+    const float4 c001 = p[plane], c101 = p[plane + 1], c011 = p[plane + A.nx], c111 = p[plane + A.nx + 1];
+    z0.A = c000;
+    z0.B = f4sub(c100, c000);
+    z0.Cc = f4sub(c010, c000);
+    z0.E = f4sub(f4sub(c110, c010), z0.B);
+    D.A = f4sub(c001, c000);
+    D.B = f4sub(f4sub(c101, c001), z0.B);
+    D.Cc = f4sub(f4sub(c011, c001), z0.Cc);
+    D.E = f4sub(f4sub(f4sub(c111, c011), f4sub(c101, c001)), z0.E);
+}
+
+// the plane dt above plane z (D = the cell's upper-minus-lower plane form)
+__device__ __forceinline__ ZP zstep(const ZP &z, float dt, const ZP &D) {
+    return ZP{f4fma(D.A, dt, z.A), f4fma(D.B, dt, z.B), f4fma(D.Cc, dt, z.Cc), f4fma(D.E, dt, z.E)};
+}
+
+// Steps k .. K-1 for h = 1/2 (two steps per cell).  A cell is held as its
+// lower plane form and the upper-minus-lower difference (32 registers); the
+// stage planes are z0 + D/4, z0 + D/2, then stepped by D/4 from there.  When
+// a stage point of any lane leaves its cell the wave takes one uniform branch
+// in which the failing lanes redo that step with gathers; the next iteration
+// reloads the cell and, if the redone step was the cell's first, runs the
+// second from the cache (the planes are formed the same way on both routes,
+// so what a lane computes never depends on the other lanes of its wave).
+template <bool TAU>
+__device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
+                                         double &ph) {
+    bool good = true;
+    while (k < K) {
+        const int iz = k >> 1;
+        const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
+        const bool inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
+        const int ix = inside ? min((int)px, a.nx - 2) : 0;
+        const int iy = inside ? min((int)py, a.ny - 2) : 0;
+        ZP z0, D;
+        load_planes(a, ix, iy, iz, z0, D);
+        const float ox = pxb - (float)ix, oy = pyb - (float)iy;
+        float mn = inside ? 0.f : -1.f, mx = 0.f;
+        State yn = ys;
+        float dph;
+        bool ok;
+        ZP zh;
+        if (!(k & 1)) {
+            const ZP zq = zstep(z0, 0.25f, D);
+            zh = zstep(z0, 0.5f, D);
+            dph = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx);
+            ok = mn >= 0.f && mx <= 1.f;
+            if (__ballot(!ok) != 0) goto redo;
+            ys = yn;
+            ph += (double)dph;
+            ++k;
+            good = good && (ys.Wd > -1.0f);
+            yn = ys;
+        } else {
+            zh = zstep(z0, 0.5f, D);
+        }
+        {
+            const ZP z3 = zstep(zh, 0.25f, D);
+            const ZP z1 = zstep(z3, 0.25f, D);
+            dph = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
+        }
+        ok = mn >= 0.f && mx <= 1.f;
+        if (__ballot(!ok) == 0) {
+            ys = yn;
+            ph += (double)dph;
+            ++k;
+            good = good && (ys.Wd > -1.0f);
+            continue;
+        }
+    redo:
+        if (!ok) {
+            yn = ys;
+            dph = rk4_step(a, pxb, pyb, (float)k * 0.5f, 0.5f, yn);
+        }
+        ys = yn;
+        ph += (double)dph;
+        ++k;
+        good = good && (ys.Wd > -1.0f);
+    }
+    return good;
 }
 
 // Steps k .. K-1 (z = k h) from the lattice, cell-cached.  Every lane loads
@@ -279,7 +381,7 @@ __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, floa
     while (k < K) {
         const int iz = k / a.m;
         const int kend = min(K, (iz + 1) * a.m);
-        const float px = pxb + ys.dpx, py = pyb + ys.dpy;
+        const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
         const bool inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
         const int ix = inside ? min((int)px, a.nx - 2) : 0;
         const int iy = inside ? min((int)py, a.ny - 2) : 0;
@@ -289,35 +391,6 @@ __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, floa
         bool redo = false, ok = true;
         State yn;
         float dph = 0.f;
-        if (a.m == 2 && k == iz * 2 && kend == k + 2) {
-            // both steps of the cell from five z-planes
-            const ZP z0{C.a, C.b, C.c, C.e};
-            float mn = inside ? 0.f : -1.f, mx = 0.f;
-            yn = ys;
-            {
-                const ZP zq = zplane(C, 0.25f), zh = zplane(C, 0.5f);
-                dph = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx);
-            }
-            ok = mn >= 0.f && mx <= 1.f;
-            if (__ballot(!ok) == 0) {
-                ys = yn;
-                ph += (double)dph;
-                ++k;
-                good = good && (ys.Wd > -1.0f);
-                yn = ys;
-                const ZP zh = zplane(C, 0.5f), z3 = zplane(C, 0.75f), z1 = zplane(C, 1.0f);
-                dph = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
-                ok = mn >= 0.f && mx <= 1.f;
-                if (__ballot(!ok) == 0) {
-                    ys = yn;
-                    ph += (double)dph;
-                    ++k;
-                    good = good && (ys.Wd > -1.0f);
-                    continue;
-                }
-            }
-            redo = true;  // step k is redone below (gathers for the lanes that left)
-        }
         while (!redo && k < kend) {
             const float tz0 = (float)(k - iz * a.m) * a.h;
             ok = inside;
@@ -346,8 +419,9 @@ __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, floa
     return good;
 }
 
-// One ray: the t-th of the launch (a.order[t] when given).
-template <bool CACHED, bool TAU>
+// One ray: the t-th of the launch (a.order[t] when given).  CACHED selects
+// the cell-cached march (M2: its h = 1/2 form, which requires a.m == 2).
+template <bool CACHED, bool M2, bool TAU>
 __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
     const int64_t r = a.order ? (int64_t)a.order[t] : t;
     const double *s = a.s0 + r;
@@ -389,7 +463,9 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
             if (CACHED) {
                 // partial first step (off-lattice start) with gathers, then the lattice
                 if (h0 > 1e-12) ph += (double)rk4_step(a, pxb, pyb, (float)Zs, (float)h0, ys);
-                if (!march_cached<TAU>(a, pxb, pyb, kstart, K, ys, ph)) st = 1;
+                if (!(M2 ? march_m2<TAU>(a, pxb, pyb, kstart, K, ys, ph)
+                         : march_cached<TAU>(a, pxb, pyb, kstart, K, ys, ph)))
+                    st = 1;
             } else {
                 if (h0 > 1e-12) ph += (double)rk4_step(a, pxb, pyb, (float)Zs, (float)h0, ys);
                 for (int k = kstart; k < K; ++k) {
@@ -400,8 +476,8 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
                     }
                 }
             }
-            px += (double)ys.dpx;
-            py += (double)ys.dpy;
+            px += (double)a.sx * (double)ys.qx;
+            py += (double)a.sy * (double)ys.qy;
             tau += a.dz * ((Zend - Zs) + (double)ys.tr);
         }
     }
@@ -429,11 +505,13 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
     if (a.status) a.status[r] = st;
 }
 
-template <bool CACHED, bool TAU>
-__global__ __launch_bounds__(256) void trace_rk4_kernel(TraceArgs a) {
+// The h = 1/2 form fits 96 VGPRs (five waves per SIMD) with one 8-byte spill
+// outside the step loop.
+template <bool CACHED, bool M2, bool TAU>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M2 ? 5 : 1))) void trace_rk4_kernel(TraceArgs a) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
-    trace_ray<CACHED, TAU>(a, t);
+    trace_ray<CACHED, M2, TAU>(a, t);
 }
 
 // ------------------------------------------------------------- ensembles
@@ -449,8 +527,8 @@ __global__ void reset_kernel(BatchHeader *h) {
     if (threadIdx.x == 0 && blockIdx.x == 0) h->counter = 0ull;
 }
 
-template <bool CACHED>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_batch_kernel(BatchHeader *hdr, const TraceArgs *__restrict__ items,
+template <bool CACHED, bool M2>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M2 ? 5 : 4))) void trace_batch_kernel(BatchHeader *hdr, const TraceArgs *__restrict__ items,
                                                           const int32_t *__restrict__ chunk_item,
                                                           const int64_t *__restrict__ chunk_start, int nchunks,
                                                           int chunk) {
@@ -463,7 +541,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void t
         if (c >= nchunks) return;  // block-uniform: every wave of the block leaves
         const TraceArgs &a = items[__builtin_amdgcn_readfirstlane(chunk_item[c])];
         const int64_t s = chunk_start[c], e = min(s + (int64_t)chunk, a.n);
-        for (int64_t t = s + threadIdx.x; t < e; t += blockDim.x) trace_ray<CACHED, false>(a, t);
+        for (int64_t t = s + threadIdx.x; t < e; t += blockDim.x) trace_ray<CACHED, M2, false>(a, t);
     }
 }
 
@@ -558,12 +636,17 @@ extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_t
     const dim3 nblk((unsigned)tt::ceil_div(n, block));
     hipStream_t st = tt::as_stream(stream);
     // the time integral is needed only for sf
-    if (opts->variant == 0 && sf)
-        hipLaunchKernelGGL((trace_rk4_kernel<true, true>), nblk, dim3(block), 0, st, a);
-    else if (opts->variant == 0)
-        hipLaunchKernelGGL((trace_rk4_kernel<true, false>), nblk, dim3(block), 0, st, a);
+    const bool m2 = m == 2;
+    if (opts->variant == 1)
+        hipLaunchKernelGGL((trace_rk4_kernel<false, false, true>), nblk, dim3(block), 0, st, a);
+    else if (m2 && sf)
+        hipLaunchKernelGGL((trace_rk4_kernel<true, true, true>), nblk, dim3(block), 0, st, a);
+    else if (m2)
+        hipLaunchKernelGGL((trace_rk4_kernel<true, true, false>), nblk, dim3(block), 0, st, a);
+    else if (sf)
+        hipLaunchKernelGGL((trace_rk4_kernel<true, false, true>), nblk, dim3(block), 0, st, a);
     else
-        hipLaunchKernelGGL((trace_rk4_kernel<false, true>), nblk, dim3(block), 0, st, a);
+        hipLaunchKernelGGL((trace_rk4_kernel<true, false, false>), nblk, dim3(block), 0, st, a);
     TT_LAUNCH_CHECK("tt_trace_rk4 launch");
     return TT_OK;
 }
@@ -658,6 +741,7 @@ extern "C" int tt_batch_prepare(const tt_batch_item *items, int32_t nitems, doub
     info->nitems = nitems;
     info->nchunks = (int32_t)nchunks;
     info->chunk = chunk;
+    info->steps_per_cell = m;
     return TT_OK;
 }
 
@@ -668,15 +752,16 @@ extern "C" int tt_batch_trace(void *workspace, const tt_batch_info *info, const 
                "tt_batch_trace: block must be 64, 128 or 256");
     TT_REQUIRE(tune->variant == 0 || tune->variant == 1, "tt_batch_trace: variant must be 0 or 1");
     TT_REQUIRE(tune->blocks_per_cu >= 0 && tune->blocks_per_cu <= 64, "tt_batch_trace: blocks_per_cu in [0, 64]");
+    TT_REQUIRE(info->steps_per_cell >= 1 && info->steps_per_cell <= 64, "tt_batch_trace: info not prepared");
+    // variant 1: gathers; variant 0: cell-cached (its h = 1/2 form when m == 2)
+    using BatchKernel = void (*)(BatchHeader *, const TraceArgs *, const int32_t *, const int64_t *, int, int);
+    const BatchKernel kern = tune->variant == 1          ? trace_batch_kernel<false, false>
+                             : info->steps_per_cell == 2 ? trace_batch_kernel<true, true>
+                                                         : trace_batch_kernel<true, false>;
     int dev = 0, ncu = 0, occ = 0;
     TT_HIP(hipGetDevice(&dev), "tt_batch_trace: hipGetDevice");
     TT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "tt_batch_trace: CU count");
-    if (tune->variant == 0)
-        TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_batch_kernel<true>, tune->block, 0),
-               "tt_batch_trace: occupancy");
-    else
-        TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_batch_kernel<false>, tune->block, 0),
-               "tt_batch_trace: occupancy");
+    TT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, tune->block, 0), "tt_batch_trace: occupancy");
     const int per_cu = tune->blocks_per_cu > 0 ? std::min(tune->blocks_per_cu, std::max(occ, 1)) : std::max(occ, 1);
     const dim3 grid((unsigned)(ncu * per_cu));
     char *w = static_cast<char *>(workspace);
@@ -687,12 +772,7 @@ extern "C" int tt_batch_trace(void *workspace, const tt_batch_info *info, const 
     const TraceArgs *items = reinterpret_cast<const TraceArgs *>(w + L.items);
     const int32_t *citem = reinterpret_cast<const int32_t *>(w + L.citem);
     const int64_t *cstart = reinterpret_cast<const int64_t *>(w + L.cstart);
-    if (tune->variant == 0)
-        hipLaunchKernelGGL(trace_batch_kernel<true>, grid, dim3(tune->block), 0, st, hdr, items, citem, cstart,
-                           info->nchunks, info->chunk);
-    else
-        hipLaunchKernelGGL(trace_batch_kernel<false>, grid, dim3(tune->block), 0, st, hdr, items, citem, cstart,
-                           info->nchunks, info->chunk);
+    hipLaunchKernelGGL(kern, grid, dim3(tune->block), 0, st, hdr, items, citem, cstart, info->nchunks, info->chunk);
     TT_LAUNCH_CHECK("tt_batch_trace launch");
     return TT_OK;
 }
