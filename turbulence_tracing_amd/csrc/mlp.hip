@@ -266,7 +266,8 @@ __global__ void standardize_kernel(const float *x, int64_t srow, int64_t scol, i
 // LDS (146 VGPRs: three blocks per CU), so each fragment is read once from L2
 // per block and once from LDS per wave for two MFMAs.  The accumulators start
 // from the bias, so the epilogue is GELU and the bf16 conversion only.
-constexpr int kInferRays = 128;
+constexpr int kInferTiles = 2;                // 16-ray tiles per wave
+constexpr int kInferRays = 64 * kInferTiles;  // rays per block of 4 waves
 constexpr int kLdsFrags = 32;  // 32 KB
 
 struct InferArgs {
@@ -288,10 +289,10 @@ __device__ __forceinline__ void stage_frags(const __bf16 *src, __bf16 *lds) {
     for (int i = 0; i < NFR * 64 / 256; ++i) d4[i * 256 + threadIdx.x] = s4[i * 256 + threadIdx.x];
 }
 
-// One hidden layer: x (KC chunks x 2 ray tiles) -> y (8 chunks x 2 tiles).
-template <int KC>
-__device__ __forceinline__ void hidden_layer(const __bf16 *w, const float *bias, __bf16 *lds, const bf16x8 (&x)[KC][2],
-                                             bf16x8 (&y)[8][2]) {
+// One hidden layer: x (KC chunks x T ray tiles) -> y (8 chunks x T tiles).
+template <int KC, int T>
+__device__ __forceinline__ void hidden_layer(const __bf16 *w, const float *bias, __bf16 *lds, const bf16x8 (&x)[KC][T],
+                                             bf16x8 (&y)[8][T]) {
     constexpr int NG = kLdsFrags / KC < 16 ? kLdsFrags / KC : 16;  // output tiles per LDS stage
     const int lane = threadIdx.x & 63, g = lane >> 4;
 #pragma unroll
@@ -304,30 +305,33 @@ __device__ __forceinline__ void hidden_layer(const __bf16 *w, const float *bias,
             // the accumulators start from the bias (lane l: features 4(l/16) + i)
             const int nt = grp * NG + tl;
             const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * nt + 4 * g);
-            f32x4 acc0 = {bb.x, bb.y, bb.z, bb.w}, acc1 = acc0;
+            f32x4 acc[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) acc[t] = f32x4{bb.x, bb.y, bb.z, bb.w};
 #pragma unroll
             for (int c = 0; c < KC; ++c) {
                 const bf16x8 a = *reinterpret_cast<const bf16x8 *>(lds + ((tl * KC + c) * 64 + lane) * 8);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][0], acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][1], acc1, 0, 0, 0);
+#pragma unroll
+                for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][t], acc[t], 0, 0, 0);
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                y[nt >> 1][0][4 * (nt & 1) + i] = (__bf16)gelu_f(acc0[i]);
-                y[nt >> 1][1][4 * (nt & 1) + i] = (__bf16)gelu_f(acc1[i]);
-            }
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int t = 0; t < T; ++t) y[nt >> 1][t][4 * (nt & 1) + i] = (__bf16)gelu_f(acc[t][i]);
         }
     }
 }
 
+// T ray tiles of 16 per wave, 4 waves per block
+template <int T>
 __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 lds[kLdsFrags * 512];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
-    const int64_t ray0 = (int64_t)blockIdx.x * kInferRays + wave * 32;
+    const int64_t ray0 = (int64_t)blockIdx.x * (64 * T) + wave * (16 * T);
     // input: lane holds features 32c + 8g + j of rays ray0 + 16t + r
-    bf16x8 x0[2][2];
+    bf16x8 x0[2][T];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < T; ++t) {
         const int64_t ray = ray0 + 16 * t + r;
         const float *row = a.feat + ray * a.srow;
 #pragma unroll
@@ -340,29 +344,33 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs a) {
                 x0[c][t][j] = (__bf16)v;
             }
     }
-    bf16x8 h1[8][2], h2[8][2];
-    hidden_layer<2>(a.w, a.bias, lds, x0, h1);
-    hidden_layer<8>(a.w + 16 * 2 * 512, a.bias + 256, lds, h1, h2);
-    hidden_layer<8>(a.w + (16 * 2 + 16 * 8) * 512, a.bias + 512, lds, h2, h1);
+    bf16x8 h1[8][T], h2[8][T];
+    hidden_layer<2, T>(a.w, a.bias, lds, x0, h1);
+    hidden_layer<8, T>(a.w + 16 * 2 * 512, a.bias + 256, lds, h1, h2);
+    hidden_layer<8, T>(a.w + (16 * 2 + 16 * 8) * 512, a.bias + 512, lds, h2, h1);
     // output tile: 16 rows (nout used), K = 256
     __syncthreads();
     stage_frags<8>(a.w + (16 * 2 + 2 * 16 * 8) * 512, lds);
     __syncthreads();
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const bf16x8 w = *reinterpret_cast<const bf16x8 *>(lds + (c * 64 + lane) * 8);
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, h1[c][0], acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, h1[c][1], acc1, 0, 0, 0);
+#pragma unroll
+        for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, h1[c][t], acc[t], 0, 0, 0);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int o = 4 * g + i;
         if (o >= a.nout) continue;
         const float b = a.bias[768 + o];
-        const int64_t ra = ray0 + r, rb = ray0 + 16 + r;
-        if (ra < a.n) a.Y[ra * a.ldy + o] = acc0[i] + b;
-        if (rb < a.n) a.Y[rb * a.ldy + o] = acc1[i] + b;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int64_t ra = ray0 + 16 * t + r;
+            if (ra < a.n) a.Y[ra * a.ldy + o] = acc[t][i] + b;
+        }
     }
 }
 
@@ -490,7 +498,7 @@ extern "C" int tt_mlp_infer(const float *feat, int64_t srow, int64_t scol, int64
     a.ldy = ldy;
     const int64_t blocks = tt::ceil_div(n, kInferRays);
     TT_REQUIRE(blocks <= 0x7fffffff, "tt_mlp_infer: too many rays");
-    hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, tt::as_stream(stream), a);
+    hipLaunchKernelGGL(mlp_infer_kernel<kInferTiles>, dim3((unsigned)blocks), dim3(256), 0, tt::as_stream(stream), a);
     TT_LAUNCH_CHECK("tt_mlp_infer launch");
     return TT_OK;
 }

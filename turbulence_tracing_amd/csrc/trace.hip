@@ -304,23 +304,38 @@ __device__ __forceinline__ ZP zstep(const ZP &z, float dt, const ZP &D) {
 // lower plane form and the upper-minus-lower difference (32 registers); the
 // stage planes are z0 + D/4, z0 + D/2, then stepped by D/4 from there.  When
 // a stage point of any lane leaves its cell the wave takes one uniform branch
-// in which the failing lanes redo that step with gathers; the next iteration
-// reloads the cell and, if the redone step was the cell's first, runs the
-// second from the cache (the planes are formed the same way on both routes,
-// so what a lane computes never depends on the other lanes of its wave).
+// in which the failing lanes redo that step with gathers; if the redone step
+// was the cell's first, the next iteration reloads the same cell and runs the
+// second step from it (the cell and its planes are formed the same way on
+// both routes, so what a lane computes never depends on the other lanes of
+// its wave).  A start between z-planes (odd k) takes one gathered step first.
 template <bool TAU>
 __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
                                          double &ph) {
     bool good = true;
+    if (k < K && (k & 1)) {
+        ph += (double)rk4_step(a, pxb, pyb, (float)k * 0.5f, 0.5f, ys);
+        good = ys.Wd > -1.0f;
+        ++k;
+    }
+    float ox = 0.f, oy = 0.f;  // entry point relative to the current cell
+    bool inside = false;       // the cell lies in the grid (x, y)
     while (k < K) {
         const int iz = k >> 1;
-        const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
-        const bool inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
-        const int ix = inside ? min((int)px, a.nx - 2) : 0;
-        const int iy = inside ? min((int)py, a.ny - 2) : 0;
+        int ix, iy;
+        if (!(k & 1)) {  // a new cell: locate it
+            const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
+            inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
+            ix = inside ? min((int)px, a.nx - 2) : 0;
+            iy = inside ? min((int)py, a.ny - 2) : 0;
+            ox = pxb - (float)ix;
+            oy = pyb - (float)iy;
+        } else {  // the cell of the redone first step (|rounding of ox| << 1/2)
+            ix = (int)rintf(pxb - ox);
+            iy = (int)rintf(pyb - oy);
+        }
         ZP z0, D;
         load_planes(a, ix, iy, iz, z0, D);
-        const float ox = pxb - (float)ix, oy = pyb - (float)iy;
         float mn = inside ? 0.f : -1.f, mx = 0.f;
         State yn = ys;
         float dph;
