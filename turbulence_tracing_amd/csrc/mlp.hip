@@ -261,14 +261,19 @@ __global__ void standardize_kernel(const float *x, int64_t srow, int64_t scol, i
 // Packed weights: per layer, fragments (tile nt, chunk c) in order
 // nt * KC + c, each 64 lanes x 8 bf16 (lane l: W[16nt + l%16][kperm(c, l/16,
 // j)]), zero-padded.  Layer shapes (NT x KC): 16 x 2, 16 x 8, 16 x 8, 1 x 8.
-// A block of 4 waves takes 128 rays (32 per wave, two 16-ray B tiles) and
-// stages the fragments of up to 4 output tiles at a time through 32 KB of
-// LDS (146 VGPRs: three blocks per CU), so each fragment is read once from L2
-// per block and once from LDS per wave for two MFMAs.  The accumulators start
-// from the bias, so the epilogue is GELU and the bf16 conversion only.
+// A block of 4 waves takes 128 rays (32 per wave, two 16-ray B tiles).  The
+// 296 weight fragments (1 KB each, all four layers in one stream) pass
+// through two 16 KB LDS buffers in chunks of 16 fragments: the copy of chunk
+// c + 1 (global -> LDS direct, no registers) is in flight while chunk c is
+// computed, and the waves of a block meet once per chunk at an LDS-only
+// barrier, so the copy's latency hides behind a chunk of MFMAs.  Each
+// fragment is read once from L2 per block and once from LDS per wave for two
+// MFMAs.  The accumulators start from the bias (kept in LDS), so the epilogue
+// is GELU and the bf16 conversion only.  160 VGPRs: three blocks per CU.
 constexpr int kInferTiles = 2;                // 16-ray tiles per wave
 constexpr int kInferRays = 64 * kInferTiles;  // rays per block of 4 waves
-constexpr int kLdsFrags = 32;  // 32 KB
+constexpr int kFragsTotal = 16 * 2 + 16 * 8 + 16 * 8 + 8;  // 296 fragments of 1 KB
+constexpr int kPipeG = 16;                    // fragments per chunk
 
 struct InferArgs {
     const float *feat;    // element (ray r, feature f) at feat[r * srow + f * scol]
@@ -281,54 +286,105 @@ struct InferArgs {
     int64_t ldy;
 };
 
-template <int NFR>
-__device__ __forceinline__ void stage_frags(const __bf16 *src, __bf16 *lds) {
-    const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-    uint4 *d4 = reinterpret_cast<uint4 *>(lds);
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// issue the copy of fragment chunk `chunk` (G fragments, fewer at the end)
+// into LDS buffer `buf`: lane l of wave v moves 16 bytes per instruction to
+// buf + (256 i + 64 v + l) * 16
+template <int G>
+__device__ __forceinline__ void copy_chunk(const __bf16 *w, int chunk, __bf16 *buf) {
+    const int nfr = kFragsTotal - chunk * G < G ? kFragsTotal - chunk * G : G;
+    if (nfr <= 0) return;
+    const uint4 *src = reinterpret_cast<const uint4 *>(w) + (size_t)chunk * G * 64;
+    uint4 *dst = reinterpret_cast<uint4 *>(buf);
+    const int wave = threadIdx.x >> 6;
 #pragma unroll
-    for (int i = 0; i < NFR * 64 / 256; ++i) d4[i * 256 + threadIdx.x] = s4[i * 256 + threadIdx.x];
+    for (int i = 0; i < G * 64 / 256; ++i)
+        if (i * 256 < nfr * 64)
+            __builtin_amdgcn_global_load_lds((global_void *)(src + i * 256 + threadIdx.x),
+                                             (lds_void *)(dst + i * 256 + wave * 64), 16, 0, 0);
 }
 
-// One hidden layer: x (KC chunks x T ray tiles) -> y (8 chunks x T tiles).
-template <int KC, int T>
-__device__ __forceinline__ void hidden_layer(const __bf16 *w, const float *bias, __bf16 *lds, const bf16x8 (&x)[KC][T],
-                                             bf16x8 (&y)[8][T]) {
-    constexpr int NG = kLdsFrags / KC < 16 ? kLdsFrags / KC : 16;  // output tiles per LDS stage
+// LDS-only workgroup barrier: the fences order LDS operations only, so the
+// barrier does not wait for this wave's global loads
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The two buffers are separate LDS arrays, so the compiler can tell a copy
+// into one from the reads of the other and does not wait for the copy first.
+template <int G>
+struct Pipe {
+    const __bf16 *w;
+    __bf16 *b0, *b1;
+    __device__ __forceinline__ __bf16 *buf(int c) const { return (c & 1) ? b1 : b0; }
+    // chunk c is about to be computed: start the copy of chunk c + 1
+    __device__ __forceinline__ void ahead(int c) const { copy_chunk<G>(w, c + 1, buf(c + 1)); }
+    // chunk c is computed: this thread's copies of chunk c + 1 have landed
+    // (vmcnt 0), then every wave has finished with chunk c and copied c + 1
+    __device__ __forceinline__ void done(int) const {
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt / lgkmcnt not waited on
+        lds_barrier();
+    }
+};
+
+// one layer of NT output tiles x KC input chunks from the fragment stream,
+// chunks C0 .. C0 + NT*KC/G - 1
+template <int KC, int T, int NT, int C0, int G, bool GELU>
+__device__ __forceinline__ void pipe_layer(const Pipe<G> &P, const float *bias, const bf16x8 (&x)[KC][T],
+                                           bf16x8 (&y)[8][T], f32x4 (&out)[T]) {
+    constexpr int NG = G / KC;  // output tiles per chunk
     const int lane = threadIdx.x & 63, g = lane >> 4;
 #pragma unroll
-    for (int grp = 0; grp < 16 / NG; ++grp) {
-        __syncthreads();
-        stage_frags<NG * KC>(w + (size_t)grp * NG * KC * 512, lds);
-        __syncthreads();
+    for (int grp = 0; grp < (NT + NG - 1) / NG; ++grp) {
+        const int c = C0 + grp;
+        P.ahead(c);
+        const __bf16 *lb = P.buf(c);
 #pragma unroll
         for (int tl = 0; tl < NG; ++tl) {
-            // the accumulators start from the bias (lane l: features 4(l/16) + i)
             const int nt = grp * NG + tl;
-            const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * nt + 4 * g);
+            if (nt >= NT) break;
             f32x4 acc[T];
+            if (GELU) {  // bias from LDS: a global load here would wait on the copies in flight
+                const float4 bb = *reinterpret_cast<const float4 *>(bias + 16 * nt + 4 * g);
 #pragma unroll
-            for (int t = 0; t < T; ++t) acc[t] = f32x4{bb.x, bb.y, bb.z, bb.w};
+                for (int t = 0; t < T; ++t) acc[t] = f32x4{bb.x, bb.y, bb.z, bb.w};
+            } else {
 #pragma unroll
-            for (int c = 0; c < KC; ++c) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(lds + ((tl * KC + c) * 64 + lane) * 8);
-#pragma unroll
-                for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[c][t], acc[t], 0, 0, 0);
+                for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
             }
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int cc = 0; cc < KC; ++cc) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(lb + ((tl * KC + cc) * 64 + lane) * 8);
 #pragma unroll
-                for (int t = 0; t < T; ++t) y[nt >> 1][t][4 * (nt & 1) + i] = (__bf16)gelu_f(acc[t][i]);
+                for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[cc][t], acc[t], 0, 0, 0);
+            }
+            if (GELU) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int t = 0; t < T; ++t) y[nt >> 1][t][4 * (nt & 1) + i] = (__bf16)gelu_f(acc[t][i]);
+            } else {
+#pragma unroll
+                for (int t = 0; t < T; ++t) out[t] = acc[t];
+            }
         }
+        P.done(c);
     }
 }
 
-// T ray tiles of 16 per wave, 4 waves per block
-template <int T>
-__global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs a) {
-    __shared__ __attribute__((aligned(16))) __bf16 lds[kLdsFrags * 512];
+template <int T, int G>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void mlp_infer_kernel(InferArgs a) {
+    __shared__ __attribute__((aligned(16))) __bf16 lds0[G * 512], lds1[G * 512];
+    __shared__ __attribute__((aligned(16))) float lbias[768];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15, g = lane >> 4;
+    const Pipe<G> P{a.w, lds0, lds1};
+    copy_chunk<G>(a.w, 0, P.buf(0));
+    for (int j = threadIdx.x; j < 768; j += 256) lbias[j] = a.bias[j];
     const int64_t ray0 = (int64_t)blockIdx.x * (64 * T) + wave * (16 * T);
-    // input: lane holds features 32c + 8g + j of rays ray0 + 16t + r
     bf16x8 x0[2][T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
@@ -344,23 +400,14 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(InferArgs a) {
                 x0[c][t][j] = (__bf16)v;
             }
     }
+    __syncthreads();  // features, biases and chunk 0 are in
     bf16x8 h1[8][T], h2[8][T];
-    hidden_layer<2, T>(a.w, a.bias, lds, x0, h1);
-    hidden_layer<8, T>(a.w + 16 * 2 * 512, a.bias + 256, lds, h1, h2);
-    hidden_layer<8, T>(a.w + (16 * 2 + 16 * 8) * 512, a.bias + 512, lds, h2, h1);
-    // output tile: 16 rows (nout used), K = 256
-    __syncthreads();
-    stage_frags<8>(a.w + (16 * 2 + 2 * 16 * 8) * 512, lds);
-    __syncthreads();
     f32x4 acc[T];
-#pragma unroll
-    for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-        const bf16x8 w = *reinterpret_cast<const bf16x8 *>(lds + (c * 64 + lane) * 8);
-#pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, h1[c][t], acc[t], 0, 0, 0);
-    }
+    constexpr int C1 = 0, C2 = C1 + 16 * 2 / G, C3 = C2 + 16 * 8 / G, C4 = C3 + 16 * 8 / G;
+    pipe_layer<2, T, 16, C1, G, true>(P, lbias, x0, h1, acc);
+    pipe_layer<8, T, 16, C2, G, true>(P, lbias + 256, h1, h2, acc);
+    pipe_layer<8, T, 16, C3, G, true>(P, lbias + 512, h2, h1, acc);
+    pipe_layer<8, T, 1, C4, G, false>(P, lbias, h1, h2, acc);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int o = 4 * g + i;
@@ -498,7 +545,8 @@ extern "C" int tt_mlp_infer(const float *feat, int64_t srow, int64_t scol, int64
     a.ldy = ldy;
     const int64_t blocks = tt::ceil_div(n, kInferRays);
     TT_REQUIRE(blocks <= 0x7fffffff, "tt_mlp_infer: too many rays");
-    hipLaunchKernelGGL(mlp_infer_kernel<kInferTiles>, dim3((unsigned)blocks), dim3(256), 0, tt::as_stream(stream), a);
+    hipLaunchKernelGGL((mlp_infer_kernel<kInferTiles, kPipeG>), dim3((unsigned)blocks), dim3(256), 0,
+                       tt::as_stream(stream), a);
     TT_LAUNCH_CHECK("tt_mlp_infer launch");
     return TT_OK;
 }
