@@ -11,7 +11,7 @@
 //
 // Counts are uint32 atomics (order-independent, bit-reproducible);
 // interferogram weights are float32 atomics.  Each workgroup first bins its
-// rays into a private LDS copy of the bin window its rays touch when that
+// 1024 rays into a private LDS copy of the bin window they touch when that
 // window is small (rays sorted by entry column land close together on the
 // detector), then flushes the non-zero bins; otherwise it adds to global
 // memory directly.
@@ -95,78 +95,134 @@ __device__ __forceinline__ int bin_of(double v, double lo, double hi, double ste
 }
 
 constexpr int kDetBlock = 256;
+constexpr int kDetRays = 4;                       // rays per thread
+constexpr int kDetBlockRays = kDetBlock * kDetRays;
 constexpr int kLdsBins = 8192;  // private window: 32 KB of uint32/float
 
+// LDS-only workgroup barrier: the fences order LDS operations only, so the
+// barrier does not wait for this wave's global atomics to complete
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ int wave_min(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// A block bins 1024 rays (4 per thread, coalesced) into each detector in
+// turn: the bin window of its hits (wave reductions, then 4 LDS atomics per
+// wave), a private LDS copy of that window when it holds at most kLdsBins
+// bins, then the non-zero bins flushed with global atomics.  The barriers
+// order LDS only, so the flush of one detector overlaps the next.
 __global__ __launch_bounds__(kDetBlock) void detect_kernel(DetArgs a) {
     __shared__ uint32_t lds[kLdsBins];
-    __shared__ int win[4];  // bx0, bx1, by0, by1 of this block's rays
-    const int64_t i = (int64_t)blockIdx.x * kDetBlock + threadIdx.x;
-    const bool live = i < a.n;
-    double x0 = 0, th0 = 0, y0 = 0, ph0 = 0, phase = 0;
-    if (live) {
-        x0 = a.rf[i];
-        th0 = a.rf[a.ld + i];
-        y0 = a.rf[2 * a.ld + i];
-        ph0 = a.rf[3 * a.ld + i];
-        if (a.phase) phase = a.phase[i];
+    __shared__ int win[4];  // bx0, bx1, by0, by1 of this block's hits
+    double x0[kDetRays], th0[kDetRays], y0[kDetRays], ph0[kDetRays], phase[kDetRays];
+    bool live[kDetRays];
+#pragma unroll
+    for (int j = 0; j < kDetRays; ++j) {
+        const int64_t i = (int64_t)blockIdx.x * kDetBlockRays + j * kDetBlock + threadIdx.x;
+        live[j] = i < a.n;
+        x0[j] = th0[j] = y0[j] = ph0[j] = phase[j] = 0.0;
+        if (live[j]) {
+            x0[j] = a.rf[i];
+            th0[j] = a.rf[a.ld + i];
+            y0[j] = a.rf[2 * a.ld + i];
+            ph0[j] = a.rf[3 * a.ld + i];
+            if (a.phase) phase[j] = a.phase[i];
+        }
+    }
+    if (threadIdx.x == 0) {
+        win[0] = 0x7fffffff;
+        win[1] = -1;
+        win[2] = 0x7fffffff;
+        win[3] = -1;
     }
     for (int k = 0; k < a.ndet; ++k) {
         const DetDev &d = a.d[k];
-        double x = x0, th = th0, y = y0, ph = ph0;
-        int bx = -1, by = -1;
-        float w = 1.0f;
-        if (live && apply_train(d, x, th, y, ph)) {
-            bx = bin_of(x, d.xmin, d.xmax, d.xstep, d.nbx);
-            by = bin_of(y, d.ymin, d.ymax, d.ystep, d.nby);
-            if (d.mode == 1) w = (float)(1.0 + cos(phase - d.tilt_k * x));
+        int bx[kDetRays], by[kDetRays];
+        float w[kDetRays];
+        int lx0 = 0x7fffffff, lx1 = -1, ly0 = 0x7fffffff, ly1 = -1;
+#pragma unroll
+        for (int j = 0; j < kDetRays; ++j) {
+            double x = x0[j], th = th0[j], y = y0[j], ph = ph0[j];
+            bx[j] = by[j] = -1;
+            w[j] = 1.0f;
+            if (live[j] && apply_train(d, x, th, y, ph)) {
+                bx[j] = bin_of(x, d.xmin, d.xmax, d.xstep, d.nbx);
+                by[j] = bin_of(y, d.ymin, d.ymax, d.ystep, d.nby);
+                if (d.mode == 1) w[j] = (float)(1.0 + cos(phase[j] - d.tilt_k * x));
+            }
+            if (bx[j] < 0 || by[j] < 0) bx[j] = by[j] = -1;
+            if (bx[j] >= 0) {
+                lx0 = min(lx0, bx[j]);
+                lx1 = max(lx1, bx[j]);
+                ly0 = min(ly0, by[j]);
+                ly1 = max(ly1, by[j]);
+            }
         }
-        const bool hit = bx >= 0 && by >= 0;
-        // window of bins this block touches
+        lx0 = wave_min(lx0);
+        lx1 = wave_max(lx1);
+        ly0 = wave_min(ly0);
+        ly1 = wave_max(ly1);
+        lds_barrier();  // win is initialised; the previous detector's LDS flush reads are done
+        if ((threadIdx.x & 63) == 0 && lx1 >= 0) {
+            atomicMin(&win[0], lx0);
+            atomicMax(&win[1], lx1);
+            atomicMin(&win[2], ly0);
+            atomicMax(&win[3], ly1);
+        }
+        lds_barrier();
+        const int wx0 = win[0], wx1 = win[1], wy0 = win[2], wy1 = win[3];
+        const int64_t area = (wx1 >= wx0) ? (int64_t)(wx1 - wx0 + 1) * (wy1 - wy0 + 1) : 0;
+        if (area > 0 && area <= kLdsBins) {
+            const int ww = wx1 - wx0 + 1;
+            for (int j = threadIdx.x; j < area; j += kDetBlock) lds[j] = 0u;
+            lds_barrier();
+#pragma unroll
+            for (int j = 0; j < kDetRays; ++j) {
+                if (bx[j] < 0) continue;
+                const int q = (by[j] - wy0) * ww + (bx[j] - wx0);
+                if (d.mode == 1)
+                    atomicAdd(reinterpret_cast<float *>(&lds[q]), w[j]);
+                else
+                    atomicAdd(&lds[q], 1u);
+            }
+            lds_barrier();
+            for (int j = threadIdx.x; j < area; j += kDetBlock) {
+                const uint32_t v = lds[j];
+                if (v == 0u) continue;
+                const int64_t gi = (int64_t)(wy0 + j / ww) * d.nbx + (wx0 + j % ww);
+                if (d.mode == 1)
+                    atomicAdd(&d.wsum[gi], __uint_as_float(v));
+                else
+                    atomicAdd(&d.counts[gi], v);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kDetRays; ++j) {
+                if (bx[j] < 0) continue;
+                const int64_t gi = (int64_t)by[j] * d.nbx + bx[j];
+                if (d.mode == 1)
+                    atomicAdd(&d.wsum[gi], w[j]);
+                else
+                    atomicAdd(&d.counts[gi], 1u);
+            }
+        }
+        lds_barrier();  // every thread has read win
         if (threadIdx.x == 0) {
             win[0] = 0x7fffffff;
             win[1] = -1;
             win[2] = 0x7fffffff;
             win[3] = -1;
         }
-        __syncthreads();
-        if (hit) {
-            atomicMin(&win[0], bx);
-            atomicMax(&win[1], bx);
-            atomicMin(&win[2], by);
-            atomicMax(&win[3], by);
-        }
-        __syncthreads();
-        const int wx0 = win[0], wx1 = win[1], wy0 = win[2], wy1 = win[3];
-        const int64_t area = (wx1 >= wx0) ? (int64_t)(wx1 - wx0 + 1) * (wy1 - wy0 + 1) : 0;
-        if (area > 0 && area <= kLdsBins) {
-            const int ww = wx1 - wx0 + 1;
-            for (int j = threadIdx.x; j < area; j += kDetBlock) lds[j] = 0u;
-            __syncthreads();
-            if (hit) {
-                const int j = (by - wy0) * ww + (bx - wx0);
-                if (d.mode == 1)
-                    atomicAdd(reinterpret_cast<float *>(&lds[j]), w);
-                else
-                    atomicAdd(&lds[j], 1u);
-            }
-            __syncthreads();
-            for (int j = threadIdx.x; j < area; j += kDetBlock) {
-                const uint32_t v = lds[j];
-                if (v == 0u) continue;
-                const int64_t g = (int64_t)(wy0 + j / ww) * d.nbx + (wx0 + j % ww);
-                if (d.mode == 1)
-                    atomicAdd(&d.wsum[g], __uint_as_float(v));
-                else
-                    atomicAdd(&d.counts[g], v);
-            }
-        } else if (hit) {
-            const int64_t g = (int64_t)by * d.nbx + bx;
-            if (d.mode == 1)
-                atomicAdd(&d.wsum[g], w);
-            else
-                atomicAdd(&d.counts[g], 1u);
-        }
-        __syncthreads();
     }
 }
 
@@ -253,7 +309,7 @@ extern "C" int tt_detect(const tt_detector *dets, int32_t ndet, const double *rf
     a.ld = ld;
     a.n = n;
     a.phase = phase;
-    hipLaunchKernelGGL(detect_kernel, dim3((unsigned)tt::ceil_div(n, kDetBlock)), dim3(kDetBlock), 0,
+    hipLaunchKernelGGL(detect_kernel, dim3((unsigned)tt::ceil_div(n, (int64_t)kDetBlockRays)), dim3(kDetBlock), 0,
                        tt::as_stream(stream), a);
     TT_LAUNCH_CHECK("tt_detect launch");
     return TT_OK;
