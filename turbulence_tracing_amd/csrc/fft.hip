@@ -119,7 +119,7 @@ __device__ __forceinline__ void dft<5>(float2 *v) {
 // at buf + p*stride).  Every thread first computes all of its butterflies
 // into registers, then all write: in-place with two barriers.
 template <int R>
-__device__ __noinline__ void pass_r(float2 *buf, int n, int stride, int B, int Ns, const float2 *tw) {
+__device__ __forceinline__ void pass_r_body(float2 *buf, int n, int stride, int B, int Ns, const float2 *tw) {
     constexpr int NB = (kElems + R - 1) / R;
     float2 v[NB][R];
     const int nbf = n / R;
@@ -154,6 +154,11 @@ __device__ __noinline__ void pass_r(float2 *buf, int n, int stride, int B, int N
         }
     }
     __syncthreads();
+}
+
+template <int R>
+__device__ __noinline__ void pass_r(float2 *buf, int n, int stride, int B, int Ns, const float2 *tw) {
+    pass_r_body<R>(buf, n, stride, B, Ns, tw);
 }
 
 // Direct DFT pass for any radix R (odd primes > 5): one output per item.
@@ -212,6 +217,19 @@ __device__ void lds_fft(float2 *buf, int stride, int B, const AxisPlan &ap, cons
             default: pass_generic(buf, ap.n, stride, B, Ns, R, tw); break;
         }
         Ns *= R;
+    }
+}
+
+// Power-of-two lengths with every size known at compile time, so the index
+// arithmetic of the passes is shifts and masks: the same radix order as
+// factorize (radix 8 while 8 divides what is left, then 4 or 2).
+template <int N, int NS>
+__device__ __forceinline__ void lds_fft_pow2(float2 *buf, const float2 *tw) {
+    if constexpr (NS < N) {
+        constexpr int rem = N / NS;
+        constexpr int R = rem % 8 == 0 ? 8 : (rem % 4 == 0 ? 4 : 2);
+        pass_r_body<R>(buf, N, N + 1, kBatch / N, NS, tw);
+        lds_fft_pow2<N, NS * R>(buf, tw);
     }
 }
 
@@ -349,15 +367,17 @@ __device__ __forceinline__ void bin_power(const PassArgs &a, int jx, int jy, int
 }
 #pragma clang fp contract(on)
 
-template <int MODE>
+// NF > 0: the pencil length is NF (a power of two) and B = kBatch / NF, both
+// compile-time constants; NF == 0: any length, from the plan.
+template <int MODE, int NF>
 __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
     extern __shared__ float2 smem[];
-    const int n = a.ap.n;
+    const int n = NF ? NF : a.ap.n;
     const int stride = n + 1;
+    const int B = NF ? kBatch / NF : a.B;
     float2 *buf = smem;
-    float2 *tw = smem + a.B * stride;
+    float2 *tw = smem + B * stride;
     for (int j = threadIdx.x; j < n; j += kThreads) tw[j] = a.tw[j];
-    const int B = a.B;
     const int total = B * n;
 
     if (MODE == kPowZ) {
@@ -373,7 +393,10 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + j] = kx < a.nxh ? a.spec[((int64_t)j * a.ny + fixed) * a.nxh + kx] : make_float2(0.f, 0.f);
         }
         __syncthreads();
-        lds_fft(buf, stride, B, a.ap, tw);
+        if constexpr (NF > 0)
+            lds_fft_pow2<NF, 1>(buf, tw);
+        else
+            lds_fft(buf, stride, B, a.ap, tw);
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g % B, j = g / B;
             const int kx = kx0 + p;
@@ -398,7 +421,10 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + k] = make_float2(row < rows ? a.in[row * n + k] : 0.f, 0.f);
         }
         __syncthreads();
-        lds_fft(buf, stride, B, a.ap, tw);
+        if constexpr (NF > 0)
+            lds_fft_pow2<NF, 1>(buf, tw);
+        else
+            lds_fft(buf, stride, B, a.ap, tw);
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g / n, k = g - p * n;
             const int64_t row = r0 + p;
@@ -421,7 +447,10 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + j] = v;
         }
         __syncthreads();
-        lds_fft(buf, stride, B, a.ap, tw);
+        if constexpr (NF > 0)
+            lds_fft_pow2<NF, 1>(buf, tw);
+        else
+            lds_fft(buf, stride, B, a.ap, tw);
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g % B, j = g / B;
             const int kx = kx0 + p;
@@ -450,7 +479,10 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + k] = v;
         }
         __syncthreads();
-        lds_fft(buf, stride, B, a.ap, tw);
+        if constexpr (NF > 0)
+            lds_fft_pow2<NF, 1>(buf, tw);
+        else
+            lds_fft(buf, stride, B, a.ap, tw);
         double ss = 0.0;
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g / n, k = g - p * n;
@@ -571,6 +603,10 @@ extern "C" int tt_fft_workspace_bytes(tt_fft_plan plan, size_t *bytes) {
 
 namespace {
 
+#define KERNELS_OF(M)                                                                                   \
+    (const void *)fft_pass_kernel<M, 0>, (const void *)fft_pass_kernel<M, 256>,                         \
+        (const void *)fft_pass_kernel<M, 512>, (const void *)fft_pass_kernel<M, 1024>
+
 // dynamic LDS above 64 KB must be enabled per kernel
 int enable_lds() {
     static int rc = -1;
@@ -579,14 +615,25 @@ int enable_lds() {
     // (98 KB + 32 KB of power-spectrum bins, within the 160 KB of a CU)
     const int want = (int)((smem_bytes(4096, pencils_per_block(4096)) + 2 * kMaxBins * sizeof(double) + 1023) /
                            1024 * 1024);
-    const void *ks[6] = {(const void *)fft_pass_kernel<kGenNative>, (const void *)fft_pass_kernel<kGenNoise>,
-                         (const void *)fft_pass_kernel<kPassY>,     (const void *)fft_pass_kernel<kPassX>,
-                         (const void *)fft_pass_kernel<kFwdX>,      (const void *)fft_pass_kernel<kPowZ>};
+    const void *ks[] = {KERNELS_OF(kGenNative), KERNELS_OF(kGenNoise), KERNELS_OF(kPassY),
+                        KERNELS_OF(kPassX),     KERNELS_OF(kFwdX),     KERNELS_OF(kPowZ)};
     for (const void *k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, want);
         if (e != hipSuccess) return rc = tt::hip_check(e, "hipFuncSetAttribute(max dynamic LDS)");
     }
     return rc = TT_OK;
+}
+
+// one pass over pencils of length n: the fixed-length kernel for 256, 512
+// and 1024, the general one otherwise
+template <int MODE>
+void launch_pass(int n, dim3 grid, size_t sm, hipStream_t st, const PassArgs &a) {
+    switch (n) {
+        case 256: hipLaunchKernelGGL((fft_pass_kernel<MODE, 256>), grid, dim3(kThreads), sm, st, a); break;
+        case 512: hipLaunchKernelGGL((fft_pass_kernel<MODE, 512>), grid, dim3(kThreads), sm, st, a); break;
+        case 1024: hipLaunchKernelGGL((fft_pass_kernel<MODE, 1024>), grid, dim3(kThreads), sm, st, a); break;
+        default: hipLaunchKernelGGL((fft_pass_kernel<MODE, 0>), grid, dim3(kThreads), sm, st, a); break;
+    }
 }
 
 int run_passes(tt_fft_plan plan, PassArgs a, bool native, void *workspace, float *out, double *sumsq,
@@ -608,9 +655,9 @@ int run_passes(tt_fft_plan plan, PassArgs a, bool native, void *workspace, float
         dim3 grid((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->ny);
         const size_t sm = smem_bytes(plan->nz, a.B);
         if (native)
-            hipLaunchKernelGGL(fft_pass_kernel<kGenNative>, grid, dim3(kThreads), sm, st, a);
+            launch_pass<kGenNative>(plan->nz, grid, sm, st, a);
         else
-            hipLaunchKernelGGL(fft_pass_kernel<kGenNoise>, grid, dim3(kThreads), sm, st, a);
+            launch_pass<kGenNoise>(plan->nz, grid, sm, st, a);
         TT_LAUNCH_CHECK("field pass Z launch");
     }
     // pass Y
@@ -619,7 +666,7 @@ int run_passes(tt_fft_plan plan, PassArgs a, bool native, void *workspace, float
     a.B = pencils_per_block(plan->ny);
     {
         dim3 grid((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->nz);
-        hipLaunchKernelGGL(fft_pass_kernel<kPassY>, grid, dim3(kThreads), smem_bytes(plan->ny, a.B), st, a);
+        launch_pass<kPassY>(plan->ny, grid, smem_bytes(plan->ny, a.B), st, a);
         TT_LAUNCH_CHECK("field pass Y launch");
     }
     // pass X
@@ -628,7 +675,7 @@ int run_passes(tt_fft_plan plan, PassArgs a, bool native, void *workspace, float
     a.B = pencils_per_block(plan->nx);
     {
         dim3 grid((unsigned)tt::ceil_div((int64_t)plan->ny * plan->nz, a.B));
-        hipLaunchKernelGGL(fft_pass_kernel<kPassX>, grid, dim3(kThreads), smem_bytes(plan->nx, a.B), st, a);
+        launch_pass<kPassX>(plan->nx, grid, smem_bytes(plan->nx, a.B), st, a);
         TT_LAUNCH_CHECK("field pass X launch");
     }
     return TT_OK;
@@ -693,23 +740,22 @@ extern "C" int tt_power_spectrum(tt_fft_plan plan, const float *field, int32_t m
     a.ap = plan->ax[0];
     a.tw = plan->tw[0];
     a.B = pencils_per_block(plan->nx);
-    hipLaunchKernelGGL(fft_pass_kernel<kFwdX>, dim3((unsigned)tt::ceil_div((int64_t)plan->ny * plan->nz, a.B)),
-                       dim3(kThreads), smem_bytes(plan->nx, a.B), st, a);
+    launch_pass<kFwdX>(plan->nx, dim3((unsigned)tt::ceil_div((int64_t)plan->ny * plan->nz, a.B)),
+                       smem_bytes(plan->nx, a.B), st, a);
     TT_LAUNCH_CHECK("power spectrum pass X launch");
     // pass Y
     a.ap = plan->ax[1];
     a.tw = plan->tw[1];
     a.B = pencils_per_block(plan->ny);
-    hipLaunchKernelGGL(fft_pass_kernel<kPassY>, dim3((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->nz),
-                       dim3(kThreads), smem_bytes(plan->ny, a.B), st, a);
+    launch_pass<kPassY>(plan->ny, dim3((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->nz),
+                        smem_bytes(plan->ny, a.B), st, a);
     TT_LAUNCH_CHECK("power spectrum pass Y launch");
     // pass Z with |F|^2 binning (sums and counts are ADDED to)
     a.ap = plan->ax[2];
     a.tw = plan->tw[2];
     a.B = pencils_per_block(plan->nz);
     const size_t sm = smem_bytes(plan->nz, a.B) + 2 * (size_t)nbins * sizeof(double);
-    hipLaunchKernelGGL(fft_pass_kernel<kPowZ>, dim3((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->ny),
-                       dim3(kThreads), sm, st, a);
+    launch_pass<kPowZ>(plan->nz, dim3((unsigned)tt::ceil_div(plan->nxh, a.B), (unsigned)plan->ny), sm, st, a);
     TT_LAUNCH_CHECK("power spectrum pass Z launch");
     return TT_OK;
 }

@@ -41,31 +41,42 @@ __device__ __forceinline__ T grad1(T sm, T s0, T sp, int i, int n, T c) {
     return (sp - sm) * c;
 }
 
+// A thread walks one (x, y) column through kZRun consecutive z-planes,
+// carrying the z-neighbours in registers: per node one new load along the
+// column and four x / y neighbour loads that hit in L1 (the block's 64 x 4
+// columns read the same plane together), and one float4 store.
+constexpr int kZRun = 16;
+
 template <typename T>
 __global__ __launch_bounds__(256) void grad_kernel(GradArgs<T> a) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
-    const int iz = blockIdx.z;
+    const int z0 = blockIdx.z * kZRun;
     if (ix >= a.nx || iy >= a.ny) return;
+    const int z1 = min(z0 + kZRun, a.nz);
     T beta = a.beta;
     if (a.sumsq) beta = (T)((double)beta / sqrt(*a.sumsq / a.count));
-    const T s0 = load_s(a, beta, ix, iy, iz);
-    const T xm = ix > 0 ? load_s(a, beta, ix - 1, iy, iz) : s0;
-    const T xp = ix < a.nx - 1 ? load_s(a, beta, ix + 1, iy, iz) : s0;
-    const T ym = iy > 0 ? load_s(a, beta, ix, iy - 1, iz) : s0;
-    const T yp = iy < a.ny - 1 ? load_s(a, beta, ix, iy + 1, iz) : s0;
-    const T zm = iz > 0 ? load_s(a, beta, ix, iy, iz - 1) : s0;
-    const T zp = iz < a.nz - 1 ? load_s(a, beta, ix, iy, iz + 1) : s0;
-    float4 o;
-    o.x = (float)grad1(xm, s0, xp, ix, a.nx, a.cx);
-    o.y = (float)grad1(ym, s0, yp, iy, a.ny, a.cy);
-    o.z = (float)grad1(zm, s0, zp, iz, a.nz, a.cz);
-    // n - 1 = -r / (1 + sqrt(1 - r)), evaluated without cancellation
-    const double r = a.alpha + (double)s0;
-    const double q = 1.0 - r;
-    const double eta = q > 0.0 ? -r / (1.0 + sqrt(q)) : -1.0;
-    o.w = (float)(eta * (double)a.phase_scale);
-    a.out[((int64_t)iz * a.ny + iy) * a.nx + ix] = o;
+    T zm = z0 > 0 ? load_s(a, beta, ix, iy, z0 - 1) : (T)0;
+    T s0 = load_s(a, beta, ix, iy, z0);
+    for (int iz = z0; iz < z1; ++iz) {
+        const T zp = iz < a.nz - 1 ? load_s(a, beta, ix, iy, iz + 1) : s0;
+        const T xm = ix > 0 ? load_s(a, beta, ix - 1, iy, iz) : s0;
+        const T xp = ix < a.nx - 1 ? load_s(a, beta, ix + 1, iy, iz) : s0;
+        const T ym = iy > 0 ? load_s(a, beta, ix, iy - 1, iz) : s0;
+        const T yp = iy < a.ny - 1 ? load_s(a, beta, ix, iy + 1, iz) : s0;
+        float4 o;
+        o.x = (float)grad1(xm, s0, xp, ix, a.nx, a.cx);
+        o.y = (float)grad1(ym, s0, yp, iy, a.ny, a.cy);
+        o.z = (float)grad1(zm, s0, zp, iz, a.nz, a.cz);
+        // n - 1 = -r / (1 + sqrt(1 - r)), evaluated without cancellation
+        const double r = a.alpha + (double)s0;
+        const double q = 1.0 - r;
+        const double eta = q > 0.0 ? -r / (1.0 + sqrt(q)) : -1.0;
+        o.w = (float)(eta * (double)a.phase_scale);
+        a.out[((int64_t)iz * a.ny + iy) * a.nx + ix] = o;
+        zm = s0;
+        s0 = zp;
+    }
 }
 
 template <typename T>
@@ -90,7 +101,7 @@ int launch(const void *field, int64_t sx, int64_t sy, int64_t sz, const tt_grid 
     a.count = (double)count;
     a.out = reinterpret_cast<float4 *>(out4);
     dim3 block(64, 4, 1);
-    dim3 grid((unsigned)tt::ceil_div(g->nx, 64), (unsigned)tt::ceil_div(g->ny, 4), (unsigned)g->nz);
+    dim3 grid((unsigned)tt::ceil_div(g->nx, 64), (unsigned)tt::ceil_div(g->ny, 4), (unsigned)tt::ceil_div(g->nz, kZRun));
     hipLaunchKernelGGL(grad_kernel<T>, grid, block, 0, st, a);
     TT_LAUNCH_CHECK("tt_grad_stencil launch");
     return TT_OK;
