@@ -220,17 +220,103 @@ __device__ void lds_fft(float2 *buf, int stride, int B, const AxisPlan &ap, cons
     }
 }
 
-// Power-of-two lengths with every size known at compile time, so the index
-// arithmetic of the passes is shifts and masks: the same radix order as
-// factorize (radix 8 while 8 divides what is left, then 4 or 2).
+// ---------------------------------------------------------------------
+// Power-of-two lengths (256, 512, 1024) with every size known at compile
+// time, so the index arithmetic is shifts and masks, and with the first and
+// the last Stockham pass in registers: the first pass reads its butterfly
+// inputs straight from the pass's loader (global memory or the generator)
+// and the last one hands its outputs straight to the storer, so a pencil
+// goes through LDS twice fewer than with LDS-to-LDS passes only.  Same radix
+// order as factorize (8 while 8 divides what is left, then 4 or 2).
 template <int N, int NS>
-__device__ __forceinline__ void lds_fft_pow2(float2 *buf, const float2 *tw) {
-    if constexpr (NS < N) {
-        constexpr int rem = N / NS;
-        constexpr int R = rem % 8 == 0 ? 8 : (rem % 4 == 0 ? 4 : 2);
-        pass_r_body<R>(buf, N, N + 1, kBatch / N, NS, tw);
-        lds_fft_pow2<N, NS * R>(buf, tw);
+constexpr int radix_at() {
+    return (N / NS) % 8 == 0 ? 8 : ((N / NS) % 4 == 0 ? 4 : 2);
+}
+
+// butterfly beta of B * NB -> (pencil p, index b); PFAST: p varies fastest
+// (pencils are columns of a row-major plane), else b (pencils are rows)
+template <int B, int NB, bool PFAST>
+__device__ __forceinline__ void split_bf(int beta, int &p, int &b) {
+    if (PFAST) {
+        p = beta % B;
+        b = beta / B;
+    } else {
+        p = beta / NB;
+        b = beta % NB;
     }
+}
+
+// first pass (Ns = 1): inputs are elements b + NB r of pencil p
+template <int N, bool PFAST, class Load>
+__device__ __forceinline__ void first_pass(float2 *buf, Load load) {
+    constexpr int R = radix_at<N, 1>(), NB = N / R, B = kBatch / N, TOT = B * NB;
+    constexpr int PER = (TOT + kThreads - 1) / kThreads;
+    float2 v[PER][R];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int beta = threadIdx.x + i * kThreads;
+        if (TOT % kThreads == 0 || beta < TOT) {
+            int p, b;
+            split_bf<B, NB, PFAST>(beta, p, b);
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[i][r] = load(p, b + NB * r);
+            dft<R>(v[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int beta = threadIdx.x + i * kThreads;
+        if (TOT % kThreads == 0 || beta < TOT) {
+            int p, b;
+            split_bf<B, NB, PFAST>(beta, p, b);
+#pragma unroll
+            for (int q = 0; q < R; ++q) buf[p * (N + 1) + R * b + q] = v[i][q];
+        }
+    }
+    __syncthreads();
+}
+
+// last pass (Ns = N / R): outputs are elements b + NS q of pencil p
+template <int N, int NS, bool PFAST, class Store>
+__device__ __forceinline__ void last_pass(const float2 *buf, const float2 *tw, Store store) {
+    constexpr int R = N / NS, B = kBatch / N, TOT = B * NS;
+    constexpr int PER = (TOT + kThreads - 1) / kThreads;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int beta = threadIdx.x + i * kThreads;
+        if (TOT % kThreads == 0 || beta < TOT) {
+            int p, b;
+            split_bf<B, NS, PFAST>(beta, p, b);
+            float2 v[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[r] = buf[p * (N + 1) + b + NS * r];
+            if (b) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[r] = cmul(v[r], tw[r * b]);
+            }
+            dft<R>(v);
+#pragma unroll
+            for (int q = 0; q < R; ++q) store(p, b + NS * q, v[q]);
+        }
+    }
+}
+
+template <int N, int NS, bool PFAST, class Store>
+__device__ __forceinline__ void rest_passes(float2 *buf, const float2 *tw, Store store) {
+    constexpr int R = radix_at<N, NS>();
+    if constexpr (NS * R == N) {
+        last_pass<N, NS, PFAST>(buf, tw, store);
+    } else {
+        pass_r_body<R>(buf, N, N + 1, kBatch / N, NS, tw);
+        rest_passes<N, NS * R, PFAST>(buf, tw, store);
+    }
+}
+
+// the whole transform of the block's B pencils: load(p, j) -> store(p, j, v)
+template <int N, bool PFAST, class Load, class Store>
+__device__ __forceinline__ void fft_pow2(float2 *buf, const float2 *tw, Load load, Store store) {
+    first_pass<N, PFAST>(buf, load);
+    rest_passes<N, radix_at<N, 1>(), PFAST>(buf, tw, store);
 }
 
 enum { kGenNative = 0, kGenNoise = 1, kPassY = 2, kPassX = 3, kFwdX = 4, kPowZ = 5 };
@@ -380,7 +466,87 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
     for (int j = threadIdx.x; j < n; j += kThreads) tw[j] = a.tw[j];
     const int total = B * n;
 
-    if (MODE == kPowZ) {
+    if constexpr (NF > 0) {
+        constexpr int BF = kBatch / NF;
+        if constexpr (MODE == kPowZ) {
+            double *hs = reinterpret_cast<double *>(tw + NF), *hc = hs + a.nbins;
+            for (int j = threadIdx.x; j < 2 * a.nbins; j += kThreads) hs[j] = 0.0;
+            const int kx0 = blockIdx.x * BF, fixed = blockIdx.y;
+            fft_pow2<NF, true>(
+                buf, tw,
+                [&](int p, int j) {
+                    const int kx = kx0 + p;
+                    return kx < a.nxh ? a.spec[((int64_t)j * a.ny + fixed) * a.nxh + kx] : make_float2(0.f, 0.f);
+                },
+                [&](int p, int j, float2 v) {
+                    const int kx = kx0 + p;
+                    if (kx < a.nxh) bin_power(a, kx, fixed, j, v.x * v.x + v.y * v.y, hs, hc);
+                });
+            __syncthreads();
+            for (int j = threadIdx.x; j < a.nbins; j += kThreads) {
+                if (hc[j] != 0.0) {
+                    atomicAdd(&a.hsum[j], hs[j]);
+                    atomicAdd(&a.hcnt[j], hc[j]);
+                }
+            }
+        } else if constexpr (MODE == kFwdX) {
+            const int64_t rows = (int64_t)a.ny * a.nz, r0 = (int64_t)blockIdx.x * BF;
+            fft_pow2<NF, false>(
+                buf, tw,
+                [&](int p, int k) { return make_float2(r0 + p < rows ? a.in[(r0 + p) * NF + k] : 0.f, 0.f); },
+                [&](int p, int k, float2 v) {
+                    if (r0 + p < rows && k < a.nxh) a.spec[(r0 + p) * a.nxh + k] = v;
+                });
+        } else if constexpr (MODE == kPassX) {
+            const int64_t rows = (int64_t)a.ny * a.nz, r0 = (int64_t)blockIdx.x * BF;
+            double ss = 0.0;
+            fft_pow2<NF, false>(
+                buf, tw,
+                [&](int p, int k) {
+                    if (r0 + p >= rows) return make_float2(0.f, 0.f);
+                    const float2 *srow = a.spec + (r0 + p) * a.nxh;
+                    if (k < a.nxh) return srow[k];
+                    const float2 c = srow[NF - k];
+                    return make_float2(c.x, -c.y);
+                },
+                [&](int p, int k, float2 v) {
+                    if (r0 + p >= rows) return;
+                    const float o = v.x * a.scale;
+                    a.out[(r0 + p) * NF + k] = o;
+                    ss += (double)o * (double)o;
+                });
+            if (a.sumsq) {
+                __shared__ double red[kThreads / 64];
+                for (int off = 32; off > 0; off >>= 1) ss += __shfl_down(ss, off, 64);
+                if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    double t = 0.0;
+                    for (int w = 0; w < kThreads / 64; ++w) t += red[w];
+                    atomicAdd(a.sumsq, t);
+                }
+            }
+        } else {  // kGenNative, kGenNoise (pencils along z) and kPassY (along y)
+            const int kx0 = blockIdx.x * BF, fixed = blockIdx.y;
+            fft_pow2<NF, true>(
+                buf, tw,
+                [&](int p, int j) {
+                    const int kx = kx0 + p;
+                    if (kx >= a.nxh) return make_float2(0.f, 0.f);
+                    if constexpr (MODE == kPassY)
+                        return a.spec[((int64_t)fixed * a.ny + j) * a.nxh + kx];
+                    else
+                        return gen<MODE>(a, kx, fixed, j);
+                },
+                [&](int p, int j, float2 v) {
+                    const int kx = kx0 + p;
+                    if (kx >= a.nxh) return;
+                    const int64_t idx = (MODE == kPassY) ? ((int64_t)fixed * a.ny + j) * a.nxh + kx
+                                                         : ((int64_t)j * a.ny + fixed) * a.nxh + kx;
+                    a.spec[idx] = v;
+                });
+        }
+    } else if (MODE == kPowZ) {
         // pencils along z of the half spectrum, then |F|^2 into a block-private
         // histogram (after the FFT's LDS) flushed with global atomics
         double *hs = reinterpret_cast<double *>(tw + n), *hc = hs + a.nbins;
@@ -393,10 +559,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + j] = kx < a.nxh ? a.spec[((int64_t)j * a.ny + fixed) * a.nxh + kx] : make_float2(0.f, 0.f);
         }
         __syncthreads();
-        if constexpr (NF > 0)
-            lds_fft_pow2<NF, 1>(buf, tw);
-        else
-            lds_fft(buf, stride, B, a.ap, tw);
+        lds_fft(buf, stride, B, a.ap, tw);
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g % B, j = g / B;
             const int kx = kx0 + p;
@@ -421,10 +584,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + k] = make_float2(row < rows ? a.in[row * n + k] : 0.f, 0.f);
         }
         __syncthreads();
-        if constexpr (NF > 0)
-            lds_fft_pow2<NF, 1>(buf, tw);
-        else
-            lds_fft(buf, stride, B, a.ap, tw);
+        lds_fft(buf, stride, B, a.ap, tw);
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g / n, k = g - p * n;
             const int64_t row = r0 + p;
@@ -447,10 +607,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + j] = v;
         }
         __syncthreads();
-        if constexpr (NF > 0)
-            lds_fft_pow2<NF, 1>(buf, tw);
-        else
-            lds_fft(buf, stride, B, a.ap, tw);
+        lds_fft(buf, stride, B, a.ap, tw);
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g % B, j = g / B;
             const int kx = kx0 + p;
@@ -479,10 +636,7 @@ __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
             buf[p * stride + k] = v;
         }
         __syncthreads();
-        if constexpr (NF > 0)
-            lds_fft_pow2<NF, 1>(buf, tw);
-        else
-            lds_fft(buf, stride, B, a.ap, tw);
+        lds_fft(buf, stride, B, a.ap, tw);
         double ss = 0.0;
         for (int g = threadIdx.x; g < total; g += kThreads) {
             const int p = g / n, k = g - p * n;
