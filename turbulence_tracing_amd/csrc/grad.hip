@@ -47,6 +47,8 @@ __device__ __forceinline__ T grad1(T sm, T s0, T sp, int i, int n, T c) {
 // columns read the same plane together), and one float4 store.
 constexpr int kZRun = 16;
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
 template <typename T>
 __global__ __launch_bounds__(256) void grad_kernel(GradArgs<T> a) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -73,7 +75,9 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs<T> a) {
         const double q = 1.0 - r;
         const double eta = q > 0.0 ? -r / (1.0 + sqrt(q)) : -1.0;
         o.w = (float)(eta * (double)a.phase_scale);
-        a.out[((int64_t)iz * a.ny + iy) * a.nx + ix] = o;
+        // streamed to HBM (nontemporal): the field is far larger than the caches
+        const f32x4 ov = {o.x, o.y, o.z, o.w};
+        __builtin_nontemporal_store(ov, reinterpret_cast<f32x4 *>(a.out + ((int64_t)iz * a.ny + iy) * a.nx + ix));
         zm = s0;
         s0 = zp;
     }
