@@ -142,7 +142,7 @@ __device__ __forceinline__ void kick_drift(const PathArgs &a, const Mid &m, floa
 }
 
 __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)tt::xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int64_t r = a.order ? (int64_t)a.order[t] : t;
     const double *s = a.s0 + r;

@@ -28,8 +28,6 @@
 
 namespace {
 
-constexpr bool kXcdRemap = true;
-
 struct TraceArgs {
     const float4 *field;
     int nx, ny, nz;
@@ -524,18 +522,9 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
 
 // The h = 1/2 form fits 96 VGPRs (five waves per SIMD) with one 8-byte spill
 // outside the step loop.
-// Workgroups are dealt to the 8 XCDs in turn (block b to XCD b % 8), and each
-// XCD has its own L2.  Block b takes ray block xcd_block(b), so every XCD
-// works through one contiguous eighth of the (column-ordered) rays and its L2
-// sees neighbouring columns only.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    const int q = nb / 8, r = nb % 8, x = b % 8;
-    return x * q + min(x, r) + b / 8;
-}
-
 template <bool CACHED, bool M2, bool TAU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M2 ? 5 : 1))) void trace_rk4_kernel(TraceArgs a) {
-    const int b = kXcdRemap ? xcd_block((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
+    const int b = tt::xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int64_t t = (int64_t)b * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     trace_ray<CACHED, M2, TAU>(a, t);

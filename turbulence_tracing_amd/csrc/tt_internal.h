@@ -78,4 +78,13 @@ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
 __device__ __forceinline__ float u01f(uint32_t x) { return ((float)(x >> 8) + 0.5f) * 5.9604644775390625e-8f; }
 __device__ __forceinline__ double u01d(uint32_t x) { return ((double)(x >> 8) + 0.5) * 5.9604644775390625e-8; }
 
+// Workgroups are dealt to the 8 XCDs in turn (block b to XCD b % 8), and each
+// XCD has its own L2.  Block b takes work block xcd_block(b, nb): every XCD
+// works through one contiguous eighth of the range, so with rays ordered by
+// entry column its L2 sees neighbouring columns only.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8;
+    return x * q + min(x, r) + b / 8;
+}
+
 }  // namespace tt
