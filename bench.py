@@ -85,16 +85,8 @@ def main():
     stream = torch.cuda.current_stream()
 
     def step(seed, ev=None):
-        p.generate_field(seed)
-        p.gradient()
-        p.beam(seed + 1)
-        p.sort()
-        if ev is not None:
-            ev[0].record(stream)
-        p.trace()
-        if ev is not None:
-            ev[1].record(stream)
-        p.detect()
+        # field + gradient and beam + ordering on two side streams, then trace and detectors
+        p.step(seed, trace_events=ev)
 
     for w in range(args.warmup):
         step(1000 + 2 * w)

@@ -83,6 +83,7 @@ class Pipeline:
         self.H_interf = torch.zeros((P, P), dtype=f32, device=device)
         self.opts = _lib.TraceOpts(cfg.h, float(np.sqrt(8.0) * cfg.L / 2), cfg.variant)
         self._dets = self._detectors()
+        self._side = None  # streams and events of step(), made on first use
 
     def _detectors(self):
         cfg = self.cfg
@@ -155,11 +156,41 @@ class Pipeline:
 
     def run(self, seed=0):
         """One realisation: field seed ``seed``, beam seed ``seed + 1``."""
-        self.generate_field(seed)
-        self.gradient()
-        self.beam(seed + 1)
-        self.sort()
+        self.step(seed)
+
+    def step(self, seed=0, trace_events=None):
+        """One realisation with the independent stages on two side streams:
+        field + gradient on one, beam + ordering on the other, then trace and
+        detectors on the current stream, which waits for both.  Back-to-back
+        calls overlap too: the next field and beam start while the detectors
+        run, and only the gradient and the ordering (which overwrite the node
+        field and the ordered rays) wait for the previous trace.
+        ``trace_events`` (two torch events) bracket the trace kernel."""
+        import torch
+        main = torch.cuda.current_stream()
+        if self._side is None:
+            self._side = (torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Event(), torch.cuda.Event(),
+                          torch.cuda.Event())
+            self._side[2].record(main)
+        sa, sb, traced, grad_done, sorted_done = self._side
+        with torch.cuda.stream(sa):
+            self.generate_field(seed)
+            sa.wait_event(traced)  # the node field is free once the previous trace is done
+            self.gradient()
+            grad_done.record(sa)
+        with torch.cuda.stream(sb):
+            self.beam(seed + 1)
+            sb.wait_event(traced)  # likewise the ordered rays
+            self.sort()
+            sorted_done.record(sb)
+        main.wait_event(grad_done)
+        main.wait_event(sorted_done)
+        if trace_events is not None:
+            trace_events[0].record(main)
         self.trace()
+        if trace_events is not None:
+            trace_events[1].record(main)
+        traced.record(main)
         self.detect()
 
     @property
