@@ -71,6 +71,8 @@ class Ensemble:
         self.info = _lib.BatchInfo()
         self.graph = None
         self._prepared_chunk = None
+        self.n_side_streams = 4
+        self._sides = None
 
     # ------------------------------------------------------------- stages
     def _items(self):
@@ -116,13 +118,39 @@ class Ensemble:
         for p in self.pipes:
             p.detect()
 
+    def _side_streams(self):
+        if self._sides is None:
+            self._sides = [torch.cuda.Stream() for _ in range(self.n_side_streams)]
+        return self._sides
+
     def step(self):
-        """One ensemble step, eager: every member's field, gradient, beam and
-        ray order, one batched trace, every member's detectors."""
-        self.generate()
-        self.order()
+        """One ensemble step: every member's field, gradient, beam and ray
+        order, one batched trace, every member's detectors.  The per-member
+        stages are small launches, so members are dealt round-robin to a few
+        side streams forked from and joined back into the current stream (in
+        a captured graph they become parallel branches); the trace runs on the
+        current stream."""
+        main = torch.cuda.current_stream()
+        sides = self._side_streams()
+        for sd in sides:
+            sd.wait_stream(main)
+        for i, (m, p) in enumerate(zip(self.members, self.pipes)):
+            with torch.cuda.stream(sides[i % len(sides)]):
+                p.generate_field(m.seed)
+                p.gradient()
+                p.beam(m.seed + 1)
+                p.cfg.brick_shift = self.tuning["brick_shift"]
+                p.sort()
+        for sd in sides:
+            main.wait_stream(sd)
         self.trace()
-        self.detect()
+        for sd in sides:
+            sd.wait_stream(main)
+        for i, p in enumerate(self.pipes):
+            with torch.cuda.stream(sides[i % len(sides)]):
+                p.detect()
+        for sd in sides:
+            main.wait_stream(sd)
 
     # ------------------------------------------------------------ tuning
     def signature(self):
