@@ -304,11 +304,12 @@ __device__ __forceinline__ ZP zstep(const ZP &z, float dt, const ZP &D) {
 // lower plane form and the upper-minus-lower difference (32 registers); the
 // stage planes are z0 + D/4, z0 + D/2, then stepped by D/4 from there.  When
 // a stage point of any lane leaves its cell the wave takes one uniform branch
-// in which the failing lanes redo that step with gathers; if the redone step
-// was the cell's first, the next iteration reloads the same cell and runs the
-// second step from it (the cell and its planes are formed the same way on
-// both routes, so what a lane computes never depends on the other lanes of
-// its wave).  A start between z-planes (odd k) takes one gathered step first.
+// in which the failing lanes redo that step with gathers.  If the redone step
+// was the cell's first, the next iteration runs the second step from the
+// cache: the lanes that redid it locate the cell they are now in, the others
+// reload their own cell (cell and planes are formed the same way on both
+// routes, so what a lane computes never depends on the other lanes of its
+// wave).  A start between z-planes (odd k) takes one gathered step first.
 template <bool TAU>
 __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
                                          double &ph) {
@@ -320,20 +321,22 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
     }
     float ox = 0.f, oy = 0.f;  // entry point relative to the current cell
     bool inside = false;       // the cell lies in the grid (x, y)
+    bool moved = false;        // this lane's last step was redone because it left its cell
     while (k < K) {
         const int iz = k >> 1;
         int ix, iy;
-        if (!(k & 1)) {  // a new cell: locate it
+        if (!(k & 1) || moved) {  // a new cell, or this lane has left its cell: locate it
             const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
             inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
             ix = inside ? min((int)px, a.nx - 2) : 0;
             iy = inside ? min((int)py, a.ny - 2) : 0;
             ox = pxb - (float)ix;
             oy = pyb - (float)iy;
-        } else {  // the cell of the redone first step (|rounding of ox| << 1/2)
+        } else {  // another lane's first step was redone: this lane's cell (|rounding of ox| << 1/2)
             ix = (int)rintf(pxb - ox);
             iy = (int)rintf(pyb - oy);
         }
+        moved = false;
         ZP z0, D;
         load_planes(a, ix, iy, iz, z0, D);
         float mn = inside ? 0.f : -1.f, mx = 0.f;
@@ -368,10 +371,11 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
             good = good && (ys.Wd > -1.0f);
             continue;
         }
-    redo:
+    redo:  // the cell is dead here; the lanes that left it redo step k with gathers
         if (!ok) {
             yn = ys;
             dph = rk4_step(a, pxb, pyb, (float)k * 0.5f, 0.5f, yn);
+            moved = true;
         }
         ys = yn;
         ph += (double)dph;
