@@ -84,20 +84,17 @@ def main():
     log(f"pipeline ready: {cfg.n}^3, {cfg.n_rays} rays")
     stream = torch.cuda.current_stream()
 
-    def step(seed, ev=None):
-        # field + gradient and beam + ordering on two side streams, then trace and detectors
-        p.step(seed, trace_events=ev)
-
-    for w in range(args.warmup):
-        step(1000 + 2 * w)
+    # field + gradient and beam + ordering on two side streams, then trace and
+    # detectors; consecutive realisations overlap (Pipeline.steps)
+    if args.warmup:
+        p.steps([1000 + 2 * w for w in range(args.warmup)])
     torch.cuda.synchronize()
     log(f"warmup done ({args.warmup} steps)")
 
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(2 * k, evs[k])
+    p.steps([2 * k for k in range(args.steps)], trace_events=evs)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps

@@ -161,37 +161,52 @@ class Pipeline:
     def step(self, seed=0, trace_events=None):
         """One realisation with the independent stages on two side streams:
         field + gradient on one, beam + ordering on the other, then trace and
-        detectors on the current stream, which waits for both.  Back-to-back
-        calls overlap too: the next field and beam start while the detectors
-        run, and only the gradient and the ordering (which overwrite the node
-        field and the ordered rays) wait for the previous trace.
-        ``trace_events`` (two torch events) bracket the trace kernel."""
+        detectors on the current stream, which waits for both.  The side
+        streams first wait for everything already queued on the current
+        stream (the previous step's detectors, and anything the caller queued
+        to read this pipeline's buffers).  ``trace_events`` (two torch events)
+        bracket the trace kernel."""
+        self.steps([seed], None if trace_events is None else [trace_events])
+
+    def steps(self, seeds, trace_events=None):
+        """Realisations for ``seeds`` back to back, each as in step(), with
+        consecutive ones overlapped: the next field and beam start while the
+        previous detectors run, and only the gradient and the ordering (which
+        overwrite the node field and the ordered rays) wait for the previous
+        trace.  Only the last realisation's results remain; nothing else may
+        read this pipeline's buffers until the call has been queued.
+        ``trace_events`` holds one pair of torch events per seed."""
         import torch
         main = torch.cuda.current_stream()
         if self._side is None:
             self._side = (torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Event(), torch.cuda.Event(),
                           torch.cuda.Event())
-            self._side[2].record(main)
         sa, sb, traced, grad_done, sorted_done = self._side
-        with torch.cuda.stream(sa):
-            self.generate_field(seed)
-            sa.wait_event(traced)  # the node field is free once the previous trace is done
-            self.gradient()
-            grad_done.record(sa)
-        with torch.cuda.stream(sb):
-            self.beam(seed + 1)
-            sb.wait_event(traced)  # likewise the ordered rays
-            self.sort()
-            sorted_done.record(sb)
-        main.wait_event(grad_done)
-        main.wait_event(sorted_done)
-        if trace_events is not None:
-            trace_events[0].record(main)
-        self.trace()
-        if trace_events is not None:
-            trace_events[1].record(main)
-        traced.record(main)
-        self.detect()
+        traced.record(main)  # everything queued so far, before the first realisation's writes
+        sa.wait_event(traced)
+        sb.wait_event(traced)
+        for i, seed in enumerate(seeds):
+            with torch.cuda.stream(sa):
+                self.generate_field(seed)
+                sa.wait_event(traced)  # the node field is free once the previous trace is done
+                self.gradient()
+                grad_done.record(sa)
+            with torch.cuda.stream(sb):
+                self.beam(seed + 1)
+                sb.wait_event(traced)  # likewise the ordered rays
+                self.sort()
+                sorted_done.record(sb)
+            main.wait_event(grad_done)
+            main.wait_event(sorted_done)
+            if trace_events is not None:
+                trace_events[i][0].record(main)
+            self.trace()
+            if trace_events is not None:
+                trace_events[i][1].record(main)
+            traced.record(main)
+            self.detect()
+        # later work on the caller's stream is ordered after the detectors;
+        # the side streams wait for it at the next call
 
     @property
     def ray_steps(self):
