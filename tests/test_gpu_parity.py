@@ -45,8 +45,9 @@ def test_field_dropin_matches_reference_golden(lib, golden, N):
 
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 12, 10), (9, 15, 7), (64, 64, 64), (30, 21, 14), (37, 11, 6),
                                    (128, 96, 80),
-                                   # the fixed-length passes (256, 512, 1024) along each axis
-                                   (10, 6, 256), (12, 256, 8), (512, 6, 10), (6, 4, 1024)])
+                                   # the fixed-length passes along each axis
+                                   (10, 6, 256), (12, 256, 8), (512, 6, 10), (6, 4, 1024),
+                                   (6, 8, 192), (384, 4, 6), (10, 128, 6)])
 def test_field_from_noise_any_shape(lib, shape):
     from turbulence_tracing_amd import turboGen
     rs = np.random.RandomState(sum(shape))
@@ -59,7 +60,9 @@ def test_field_from_noise_any_shape(lib, shape):
 
 @pytest.mark.parametrize("shape,beta,aniso", [((32, 32, 32), -11 / 3, 1.0), ((24, 20, 16), -3.0, 0.5),
                                               ((17, 19, 23), -3.5, 2.0), ((96, 64, 48), -11 / 3, 1.3),
-                                              ((256, 12, 10), -11 / 3, 1.0), ((10, 8, 512), -3.0, 1.5)])
+                                              ((256, 12, 10), -11 / 3, 1.0), ((10, 8, 512), -3.0, 1.5),
+                                              ((192, 10, 8), -11 / 3, 0.8), ((8, 6, 384), -3.2, 1.0),
+                                              ((6, 128, 10), -11 / 3, 1.0)])
 def test_native_field_matches_philox_twin(lib, shape, beta, aniso):
     from turbulence_tracing_amd import turboGen
     f, rms = turboGen.gaussian_field(shape, beta=beta, aniso=aniso, seed=77)

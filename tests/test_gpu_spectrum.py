@@ -12,7 +12,8 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(16, 16, 16), (15, 12, 9), (33, 20, 17), (64, 48, 40)]
 
 
-@pytest.mark.parametrize("shape", SHAPES + [(256, 6, 10), (6, 8, 512), (12, 256, 6)])
+@pytest.mark.parametrize("shape", SHAPES + [(256, 6, 10), (6, 8, 512), (12, 256, 6), (192, 6, 8), (8, 384, 6),
+                                            (6, 10, 128)])
 def test_binned_power_matches_oracle(lib, shape):
     from turbulence_tracing_amd.calculate_spectrum_3d import power_bins
     rs = np.random.RandomState(sum(shape))

@@ -221,16 +221,26 @@ __device__ void lds_fft(float2 *buf, int stride, int B, const AxisPlan &ap, cons
 }
 
 // ---------------------------------------------------------------------
-// Power-of-two lengths (256, 512, 1024) with every size known at compile
-// time, so the index arithmetic is shifts and masks, and with the first and
+// Fixed lengths (128, 192, 256, 384, 512, 1024: powers of two times at most
+// one 3) with every size known at compile time, so the index arithmetic is shifts and masks, and with the first and
 // the last Stockham pass in registers: the first pass reads its butterfly
 // inputs straight from the pass's loader (global memory or the generator)
 // and the last one hands its outputs straight to the storer, so a pencil
 // goes through LDS twice fewer than with LDS-to-LDS passes only.  Same radix
-// order as factorize (8 while 8 divides what is left, then 4 or 2).
+// order as factorize (8 while 8 divides what is left, then 4 or 2, then the
+// odd prime).
 template <int N, int NS>
 constexpr int radix_at() {
-    return (N / NS) % 8 == 0 ? 8 : ((N / NS) % 4 == 0 ? 4 : 2);
+    int m = N / NS, e = 0;
+    while (m % 2 == 0) {
+        m /= 2;
+        ++e;
+    }
+    if (e >= 3) return 8;
+    if (e == 2) return 4;
+    if (e == 1) return 2;
+    for (int p = 3;; p += 2)
+        if (m % p == 0) return p;
 }
 
 // butterfly beta of B * NB -> (pencil p, index b); PFAST: p varies fastest
@@ -461,7 +471,7 @@ __device__ __forceinline__ void bin_power(const PassArgs &a, int jx, int jy, int
 }
 #pragma clang fp contract(on)
 
-// NF > 0: the pencil length is NF (a power of two) and B = kBatch / NF, both
+// NF > 0: the pencil length is NF (one of the fixed lengths) and B = kBatch / NF, both
 // compile-time constants; NF == 0: any length, from the plan.
 template <int MODE, int NF>
 __global__ __launch_bounds__(kThreads) void fft_pass_kernel(PassArgs a) {
@@ -766,8 +776,10 @@ extern "C" int tt_fft_workspace_bytes(tt_fft_plan plan, size_t *bytes) {
 namespace {
 
 #define KERNELS_OF(M)                                                                                   \
-    (const void *)fft_pass_kernel<M, 0>, (const void *)fft_pass_kernel<M, 256>,                         \
-        (const void *)fft_pass_kernel<M, 512>, (const void *)fft_pass_kernel<M, 1024>
+    (const void *)fft_pass_kernel<M, 0>, (const void *)fft_pass_kernel<M, 128>,                         \
+        (const void *)fft_pass_kernel<M, 192>, (const void *)fft_pass_kernel<M, 256>,                   \
+        (const void *)fft_pass_kernel<M, 384>, (const void *)fft_pass_kernel<M, 512>,                   \
+        (const void *)fft_pass_kernel<M, 1024>
 
 // dynamic LDS above 64 KB must be enabled per kernel
 int enable_lds() {
@@ -786,12 +798,15 @@ int enable_lds() {
     return rc = TT_OK;
 }
 
-// one pass over pencils of length n: the fixed-length kernel for 256, 512
-// and 1024, the general one otherwise
+// one pass over pencils of length n: the fixed-length kernel for 128, 192,
+// 256, 384, 512 and 1024, the general one otherwise
 template <int MODE>
 void launch_pass(int n, dim3 grid, size_t sm, hipStream_t st, const PassArgs &a) {
     switch (n) {
+        case 128: hipLaunchKernelGGL((fft_pass_kernel<MODE, 128>), grid, dim3(kThreads), sm, st, a); break;
+        case 192: hipLaunchKernelGGL((fft_pass_kernel<MODE, 192>), grid, dim3(kThreads), sm, st, a); break;
         case 256: hipLaunchKernelGGL((fft_pass_kernel<MODE, 256>), grid, dim3(kThreads), sm, st, a); break;
+        case 384: hipLaunchKernelGGL((fft_pass_kernel<MODE, 384>), grid, dim3(kThreads), sm, st, a); break;
         case 512: hipLaunchKernelGGL((fft_pass_kernel<MODE, 512>), grid, dim3(kThreads), sm, st, a); break;
         case 1024: hipLaunchKernelGGL((fft_pass_kernel<MODE, 1024>), grid, dim3(kThreads), sm, st, a); break;
         default: hipLaunchKernelGGL((fft_pass_kernel<MODE, 0>), grid, dim3(kThreads), sm, st, a); break;
