@@ -260,7 +260,7 @@ def surrogate_large(run, cfg, plain_ms):
     del feat, resid
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    blocks = block_fields(p.field4, p.grid, k)
+    blocks = block_fields(p.field4, p.grid, k) if k > 1 else None  # k = 1 reads the node field
     first = {}
 
     def grab(w, n):

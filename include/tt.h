@@ -285,7 +285,9 @@ int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t k, float *
  * y, cells), the phase, U0, V0, W0 - 1, the entry-cell fractions x, y, the x
  * and y column changes between steps, the summed second-order proxies
  * (dGx/dx Gx + dGx/dy Gy) / W^2 and (dGy/dx Gx + dGy/dy Gy) / W^2, and the
- * path length in cells. */
+ * path length in cells.  With k = 1, GM4 may be NULL: the steps then take
+ * a cell's G and M from its two node planes of field4 (the values
+ * tt_field_blocks stores for k = 1), so no block fields are needed. */
 int tt_path_features(const float *field4, const float *GM4, int32_t k, const tt_grid *grid, const double *s0,
                      int64_t ld, int64_t n, const int32_t *order, float *feat, int64_t ldf, double *base, int64_t ldb,
                      void *stream);

@@ -146,7 +146,7 @@ def small_cube(n=33, seed=3):
     return x, 1e24 * (1 + 0.1 * d)
 
 
-@pytest.mark.parametrize("k,inside", [(1, False), (4, False), (16, False), (4, True), (16, True)])
+@pytest.mark.parametrize("k,inside", [(1, False), (4, False), (16, False), (1, True), (4, True), (16, True)])
 def test_path_features_match_oracle(lib, k, inside):
     """Block path estimate and features against the FP64 oracle on a
     turbulent 33^3 cube; ``inside`` starts the rays inside the grid off a
@@ -174,6 +174,47 @@ def test_path_features_match_oracle(lib, k, inside):
     f = feat.cpu().numpy()
     scale = np.abs(fo).max(axis=0) + 1e-30
     assert (np.abs(f - fo).max(axis=0) / scale).max() < 1e-3
+
+
+@pytest.mark.parametrize("inside", [False, True])
+def test_path_k1_node_field_matches_block_fields(lib, inside):
+    """At k = 1 the estimate reads the node field (no block fields) and
+    interpolates the sums and differences of the two node planes; it must
+    agree with the k = 1 block-field path to float rounding, on a wide beam
+    whose rays change column many times (a ray within rounding of a column
+    boundary may count one column change more or less)."""
+    import torch
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    from turbulence_tracing_amd.surrogate import block_fields, path_estimate
+    x, ne = small_cube(n=41, seed=5)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne * 20)
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(20000, 4e-3, 0.1, seed=11)
+    if inside:
+        s0[2] = -5e-3 + 7.61 * (x[1] - x[0])
+    blocks = block_fields(cube.field4, cube.grid, 1)
+    b_blk, f_blk = path_estimate(cube.field4, cube.grid, s0, k=1, blocks=blocks)
+    b_node, f_node = path_estimate(cube.field4, cube.grid, s0, k=1)
+    bb, bn = b_blk.cpu().numpy(), b_node.cpu().numpy()
+    fin = np.isfinite(bb).all(axis=0)
+    assert fin.all() and np.isfinite(bn).all()
+    dx = x[1] - x[0]
+    assert np.abs(bn[[0, 2]] - bb[[0, 2]]).max() < 1e-5 * dx
+    assert np.abs(bn[[1, 3]] - bb[[1, 3]]).max() < 1e-5 * np.abs(bb[[1, 3]]).max()
+    assert np.abs(bn[4] - bb[4]).max() < 1e-5 * np.abs(bb[4]).max()
+    fb, fn = f_blk.cpu().numpy(), f_node.cpu().numpy()
+    cross = [2 * 16 + 11, 2 * 16 + 12]
+    other = [q for q in range(fb.shape[1]) if q not in cross]
+    scale = np.abs(fb[:, other]).max(axis=0) + 1e-30
+    assert (np.abs(fn[:, other] - fb[:, other]).max(axis=0) / scale).max() < 1e-4
+    dc = np.abs(fn[:, cross] - fb[:, cross])
+    assert dc.max() <= 1 and (dc > 0).sum() <= 10
+    assert fb[:, cross[0]].max() > 3  # column changes along the path
+    from turbulence_tracing_amd import _lib
+    with pytest.raises(_lib.TTError, match="k > 1"):
+        lib.tt_path_features(cube.field4.data_ptr(), None, 4, C.byref(cube.grid), s0.data_ptr(), s0.stride(0), 10,
+                             None, f_node.data_ptr(), f_node.stride(1), b_node.data_ptr(), b_node.shape[1], None)
 
 
 def test_surrogate_gain_and_fallback(lib):

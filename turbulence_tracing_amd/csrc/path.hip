@@ -14,7 +14,9 @@
 // path, and the position gets -s M_x / W, because the kick is spread over the
 // block instead of applied at its centre.  A ray entering inside the grid
 // first takes single-cell steps from the node field up to the next block
-// boundary.  The full RK4 takes 8 field evaluations per z-cell (two steps of
+// boundary.  For k = 1 the steps can read the node field directly (a cell's
+// G and M follow from its two node planes): half the bytes per step of the
+// block fields, and no block fields to build.  The full RK4 takes 8 field evaluations per z-cell (two steps of
 // four stages); this takes one per k cells.
 //
 // Outputs per ray: base = [x, theta, y, phi, phase] (float64, the rf layout
@@ -141,6 +143,94 @@ __device__ __forceinline__ void kick_drift(const PathArgs &a, const Mid &m, floa
     S.ciy = m.iy;
 }
 
+constexpr float kTwelfth = 1.0f / 12.0f;
+
+// One drift-kick-drift step over a block of w cells starting at cell b0,
+// from the {G, M} values of the four columns around the mid-step position.
+__device__ __forceinline__ void block_step(const PathArgs &a, const Mid &m, const float4 (&Gc)[4],
+                                           const float4 (&Mc)[4], float w, int b0, int ncell, PathState &S, float *f) {
+    const float4 G = bilerp(Gc, m.tx, m.ty);
+    const float4 Mdx = lerp4(sub4(Mc[1], Mc[0]), sub4(Mc[3], Mc[2]), m.ty);
+    const float4 Mdy = lerp4(sub4(Mc[2], Mc[0]), sub4(Mc[3], Mc[1]), m.tx);
+    const float mvx = fmaf(m.ty, fmaf(m.tx, Mc[3].x - Mc[2].x, Mc[2].x) - fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x),
+                           fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x));
+    const float mvy = fmaf(m.ty, fmaf(m.tx, Mc[3].y - Mc[2].y, Mc[2].y) - fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y),
+                           fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y));
+    const float vx = a.sx * S.U * m.iW, vy = a.sy * S.V * m.iW;
+    float4 g = make_float4(fmaf(vy, Mdy.x, fmaf(vx, Mdx.x, G.x)), fmaf(vy, Mdy.y, fmaf(vx, Mdx.y, G.y)),
+                           fmaf(vy, Mdy.z, fmaf(vx, Mdx.z, G.z)), fmaf(vy, Mdy.w, fmaf(vx, Mdx.w, G.w)));
+    // second-order proxies from the slopes of G
+    const float Gdxx = fmaf(m.ty, (Gc[3].x - Gc[2].x) - (Gc[1].x - Gc[0].x), Gc[1].x - Gc[0].x);
+    const float Gdyx = fmaf(m.tx, (Gc[3].x - Gc[1].x) - (Gc[2].x - Gc[0].x), Gc[2].x - Gc[0].x);
+    const float Gdxy = fmaf(m.ty, (Gc[3].y - Gc[2].y) - (Gc[1].y - Gc[0].y), Gc[1].y - Gc[0].y);
+    const float Gdyy = fmaf(m.tx, (Gc[3].y - Gc[1].y) - (Gc[2].y - Gc[0].y), Gc[2].y - Gc[0].y);
+    const float iW2 = m.iW * m.iW;
+    if (m.inb) {
+        S.qx = fmaf(fmaf(Gdxx, G.x, Gdyx * G.y), iW2, S.qx);
+        S.qy = fmaf(fmaf(Gdxy, G.x, Gdyy * G.y), iW2, S.qy);
+    } else {
+        g = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    kick_drift(a, m, g, w, m.inb ? mvx : 0.f, m.inb ? mvy : 0.f, b0, ncell, S, f);
+}
+
+template <int J>
+__device__ __forceinline__ float comp(const float4 &v) {
+    return J == 0 ? v.x : J == 1 ? v.y : J == 2 ? v.z : v.w;
+}
+
+// Value and x / y slopes at (tx, ty) of the bilinear patch through corners
+// c0 (0, 0), c1 (1, 0), c2 (0, 1), c3 (1, 1).
+struct Patch {
+    float v, dx, dy;
+};
+__device__ __forceinline__ Patch patch(float c0, float c1, float c2, float c3, float tx, float ty) {
+    const float d01 = c1 - c0, d23 = c3 - c2;
+    const float lo = fmaf(tx, d01, c0), hi = fmaf(tx, d23, c2);
+    Patch p;
+    p.dy = hi - lo;
+    p.v = fmaf(ty, p.dy, lo);
+    p.dx = fmaf(ty, d23 - d01, d01);
+    return p;
+}
+// patches of component J of the corner sums lo + hi and differences hi - lo
+template <int J>
+__device__ __forceinline__ Patch sum_patch(const float4 (&lo)[4], const float4 (&hi)[4], float tx, float ty) {
+    return patch(comp<J>(lo[0]) + comp<J>(hi[0]), comp<J>(lo[1]) + comp<J>(hi[1]), comp<J>(lo[2]) + comp<J>(hi[2]),
+                 comp<J>(lo[3]) + comp<J>(hi[3]), tx, ty);
+}
+template <int J>
+__device__ __forceinline__ Patch dif_patch(const float4 (&lo)[4], const float4 (&hi)[4], float tx, float ty) {
+    return patch(comp<J>(hi[0]) - comp<J>(lo[0]), comp<J>(hi[1]) - comp<J>(lo[1]), comp<J>(hi[2]) - comp<J>(lo[2]),
+                 comp<J>(hi[3]) - comp<J>(lo[3]), tx, ty);
+}
+
+// One step over the single cell c from its node planes lo (c) and hi
+// (c + 1) at the four columns around the mid-step position: the block step
+// with G = (lo + hi) / 2 and M = (hi - lo) / 12, both interpolated
+// bilinearly (so the sums and differences are interpolated once, and the
+// factors 1/2 and 1/12 applied to the interpolated values).
+__device__ __forceinline__ void node_step(const PathArgs &a, const Mid &m, const float4 (&lo)[4],
+                                          const float4 (&hi)[4], int c, int ncell, PathState &S, float *f) {
+    const Patch s0 = sum_patch<0>(lo, hi, m.tx, m.ty), s1 = sum_patch<1>(lo, hi, m.tx, m.ty);
+    const Patch s2 = sum_patch<2>(lo, hi, m.tx, m.ty), s3 = sum_patch<3>(lo, hi, m.tx, m.ty);
+    const Patch d0 = dif_patch<0>(lo, hi, m.tx, m.ty), d1 = dif_patch<1>(lo, hi, m.tx, m.ty);
+    const Patch d2 = dif_patch<2>(lo, hi, m.tx, m.ty), d3 = dif_patch<3>(lo, hi, m.tx, m.ty);
+    // g = G + v . grad(M) = (s + (v / 6) . grad(d)) / 2
+    const float vx6 = a.sx * S.U * m.iW * (1.0f / 6.0f), vy6 = a.sy * S.V * m.iW * (1.0f / 6.0f);
+    float4 g = make_float4(0.5f * fmaf(vy6, d0.dy, fmaf(vx6, d0.dx, s0.v)), 0.5f * fmaf(vy6, d1.dy, fmaf(vx6, d1.dx, s1.v)),
+                           0.5f * fmaf(vy6, d2.dy, fmaf(vx6, d2.dx, s2.v)), 0.5f * fmaf(vy6, d3.dy, fmaf(vx6, d3.dx, s3.v)));
+    // second-order proxies (dGx/dx Gx + dGx/dy Gy) / W^2 with G = s / 2
+    const float iW2q = 0.25f * m.iW * m.iW;
+    if (m.inb) {
+        S.qx = fmaf(fmaf(s0.dx, s0.v, s0.dy * s1.v), iW2q, S.qx);
+        S.qy = fmaf(fmaf(s1.dx, s0.v, s1.dy * s1.v), iW2q, S.qy);
+    } else {
+        g = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    kick_drift(a, m, g, 1.0f, m.inb ? d0.v * kTwelfth : 0.f, m.inb ? d1.v * kTwelfth : 0.f, c, ncell, S, f);
+}
+
 __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
     const int64_t t = (int64_t)tt::xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
@@ -193,46 +283,52 @@ __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
         g = m.inb ? make_float4(w * g.x, w * g.y, w * g.z, w * g.w) : make_float4(0.f, 0.f, 0.f, 0.f);
         kick_drift(a, m, g, w, 0.f, 0.f, c, ncell, S, f);
     }
-    // z-blocks: a column's blocks are consecutive in memory, so a wave's
-    // successive steps read neighbouring bytes of the same few columns
-    const int64_t cstride = 2 * (int64_t)a.nb;  // float4 per column
-    for (int bb = bstart / a.k; bb < a.nb && bstart < ncell; ++bb) {
-        const int b0 = bb * a.k;
-        const float w = (float)(min(ncell, b0 + a.k) - b0);
-        const Mid m = half_drift(a, S, pxb, pyb, w);
-        const float4 *q = a.GM + ((int64_t)m.iy * a.nx + m.ix) * cstride + 2 * bb;
-        float4 Gc[4], Mc[4];
-        Gc[0] = q[0];
-        Mc[0] = q[1];
-        Gc[1] = q[cstride];
-        Mc[1] = q[cstride + 1];
-        Gc[2] = q[a.nx * cstride];
-        Mc[2] = q[a.nx * cstride + 1];
-        Gc[3] = q[(a.nx + 1) * cstride];
-        Mc[3] = q[(a.nx + 1) * cstride + 1];
-        const float4 G = bilerp(Gc, m.tx, m.ty);
-        const float4 Mdx = lerp4(sub4(Mc[1], Mc[0]), sub4(Mc[3], Mc[2]), m.ty);
-        const float4 Mdy = lerp4(sub4(Mc[2], Mc[0]), sub4(Mc[3], Mc[1]), m.tx);
-        const float mvx = fmaf(m.ty, fmaf(m.tx, Mc[3].x - Mc[2].x, Mc[2].x) - fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x),
-                               fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x));
-        const float mvy = fmaf(m.ty, fmaf(m.tx, Mc[3].y - Mc[2].y, Mc[2].y) - fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y),
-                               fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y));
-        const float vx = a.sx * S.U * m.iW, vy = a.sy * S.V * m.iW;
-        float4 g = make_float4(fmaf(vy, Mdy.x, fmaf(vx, Mdx.x, G.x)), fmaf(vy, Mdy.y, fmaf(vx, Mdx.y, G.y)),
-                               fmaf(vy, Mdy.z, fmaf(vx, Mdx.z, G.z)), fmaf(vy, Mdy.w, fmaf(vx, Mdx.w, G.w)));
-        // second-order proxies from the slopes of G
-        const float Gdxx = fmaf(m.ty, (Gc[3].x - Gc[2].x) - (Gc[1].x - Gc[0].x), Gc[1].x - Gc[0].x);
-        const float Gdyx = fmaf(m.tx, (Gc[3].x - Gc[1].x) - (Gc[2].x - Gc[0].x), Gc[2].x - Gc[0].x);
-        const float Gdxy = fmaf(m.ty, (Gc[3].y - Gc[2].y) - (Gc[1].y - Gc[0].y), Gc[1].y - Gc[0].y);
-        const float Gdyy = fmaf(m.tx, (Gc[3].y - Gc[1].y) - (Gc[2].y - Gc[0].y), Gc[2].y - Gc[0].y);
-        const float iW2 = m.iW * m.iW;
-        if (m.inb) {
-            S.qx = fmaf(fmaf(Gdxx, G.x, Gdyx * G.y), iW2, S.qx);
-            S.qy = fmaf(fmaf(Gdxy, G.x, Gdyy * G.y), iW2, S.qy);
-        } else {
-            g = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!a.GM) {
+        // k = 1 without block fields: a cell's G and M follow from its two
+        // node planes, G = (lo + hi) / 2 and M = (hi - lo) / 12 (what
+        // block_kernel stores for k = 1), so each step reads the node field:
+        // the upper plane's four corners, and the lower plane's only when the
+        // ray has changed column (else it is the last step's upper plane).
+        // Half the bytes per step of the block fields, no block fields to
+        // build; loading the next upper plane one step ahead was slower
+        // (more registers, fewer waves).
+        const int64_t plane = (int64_t)a.nx * a.ny;
+        int qx = -1, qy = -1;
+        float4 lo[4];
+        for (int c = bstart; c < ncell; ++c) {
+            const Mid m = half_drift(a, S, pxb, pyb, 1.0f);
+            const float4 *p = a.field + ((int64_t)(c + 1) * a.ny + m.iy) * a.nx + m.ix;
+            float4 hi[4];
+            load_corners(p, a.nx, hi);
+            if (m.ix != qx || m.iy != qy) {
+                load_corners(p - plane, a.nx, lo);
+                qx = m.ix;
+                qy = m.iy;
+            }
+            node_step(a, m, lo, hi, c, ncell, S, f);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) lo[q] = hi[q];
         }
-        kick_drift(a, m, g, w, m.inb ? mvx : 0.f, m.inb ? mvy : 0.f, b0, ncell, S, f);
+    } else {
+        // z-blocks: a column's blocks are consecutive in memory, so a wave's
+        // successive steps read neighbouring bytes of the same few columns
+        const int64_t cstride = 2 * (int64_t)a.nb;  // float4 per column
+        for (int bb = bstart / a.k; bb < a.nb && bstart < ncell; ++bb) {
+            const int b0 = bb * a.k;
+            const float w = (float)(min(ncell, b0 + a.k) - b0);
+            const Mid m = half_drift(a, S, pxb, pyb, w);
+            const float4 *q = a.GM + ((int64_t)m.iy * a.nx + m.ix) * cstride + 2 * bb;
+            float4 Gc[4], Mc[4];
+            Gc[0] = q[0];
+            Mc[0] = q[1];
+            Gc[1] = q[cstride];
+            Mc[1] = q[cstride + 1];
+            Gc[2] = q[a.nx * cstride];
+            Mc[2] = q[a.nx * cstride + 1];
+            Gc[3] = q[(a.nx + 1) * cstride];
+            Mc[3] = q[(a.nx + 1) * cstride + 1];
+            block_step(a, m, Gc, Mc, w, b0, ncell, S, f);
+        }
     }
     if (S.seg < kSeg) {
         f[S.seg * a.ldf] = S.segx;
@@ -298,11 +394,10 @@ __global__ void block_kernel(const float4 *__restrict__ f, float4 *__restrict__ 
             m.w = fmaf(zc, mid.w, m.w);
             lo = hi;
         }
-        const float s12 = 1.0f / 12.0f;
-        m.x = fmaf(lo.x - first.x, s12, m.x);
-        m.y = fmaf(lo.y - first.y, s12, m.y);
-        m.z = fmaf(lo.z - first.z, s12, m.z);
-        m.w = fmaf(lo.w - first.w, s12, m.w);
+        m.x = fmaf(lo.x - first.x, kTwelfth, m.x);
+        m.y = fmaf(lo.y - first.y, kTwelfth, m.y);
+        m.z = fmaf(lo.z - first.z, kTwelfth, m.z);
+        m.w = fmaf(lo.w - first.w, kTwelfth, m.w);
         out[2 * bb] = g;
         out[2 * bb + 1] = m;
     }
@@ -371,7 +466,8 @@ extern "C" int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t
 extern "C" int tt_path_features(const float *field4, const float *GM4, int32_t k, const tt_grid *grid, const double *s0,
                                 int64_t ld, int64_t n, const int32_t *order, float *feat, int64_t ldf, double *base,
                                 int64_t ldb, void *stream) {
-    TT_REQUIRE(field4 && GM4 && grid && s0 && feat && base, "tt_path_features: null pointer");
+    TT_REQUIRE(field4 && grid && s0 && feat && base, "tt_path_features: null pointer");
+    TT_REQUIRE(GM4 || k == 1, "tt_path_features: block fields GM4 are required for k > 1");
     TT_REQUIRE(n >= 0 && ld >= n && ldb >= n && ldf >= n, "tt_path_features: bad sizes");
     TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "tt_path_features: grid needs >= 2 nodes per axis");
     TT_REQUIRE(grid->dx > 0 && grid->dy > 0 && grid->dz > 0, "tt_path_features: spacing must be > 0");

@@ -228,15 +228,16 @@ def block_fields(field4, grid, k=16):
 def path_estimate(field4, grid, s0, n=None, k=16, blocks=None, order=None, base=None, feat=None):
     """Cheap path estimate base [5, n] (fp64) and features [n, 48] (fp32,
     a feature-major view: strides (1, n)) over z-blocks of k cells.  ``blocks`` = block_fields(field4, grid, k)
-    (built here when not given); ``order`` (int32, optional) is the
+    (built here when not given for k > 1; for k = 1 the steps read the node
+    field itself unless blocks are given, with the same result); ``order`` (int32, optional) is the
     processing order of the rays; ``base`` / ``feat`` (optional) receive the
     results (contiguous, of those shapes)."""
     lib = _lib.load()
     n = s0.shape[1] if n is None else n
     ld = s0.stride(0)
     dev = s0.device
-    GM = blocks if blocks is not None else block_fields(field4, grid, k)
-    if GM.shape[2] != -(-(grid.nz - 1) // k):
+    GM = blocks if blocks is not None or k == 1 else block_fields(field4, grid, k)
+    if GM is not None and GM.shape[2] != -(-(grid.nz - 1) // k):
         raise ValueError(f"path_estimate: blocks were built for another block size than k={k}")
     if base is None:
         base = torch.empty((5, n), dtype=torch.float64, device=dev)
@@ -420,7 +421,7 @@ class Surrogate:
         main = torch.cuda.current_stream()
         t0 = torch.cuda.Event(enable_timing=True)
         t0.record(main)
-        if blocks is None:
+        if blocks is None and self.k > 1:
             blocks = block_fields(field4, grid, self.k)
         rf = torch.empty((4, n), dtype=torch.float64, device=dev)
         phase = torch.empty(n, dtype=torch.float64, device=dev)
