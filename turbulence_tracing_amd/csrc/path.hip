@@ -54,9 +54,6 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
     return make_float4(fmaf(t, b.x - a.x, a.x), fmaf(t, b.y - a.y, a.y), fmaf(t, b.z - a.z, a.z),
                        fmaf(t, b.w - a.w, a.w));
 }
-__device__ __forceinline__ float4 sub4(float4 a, float4 b) {
-    return make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
-}
 __device__ __forceinline__ float4 bilerp(const float4 *c, float tx, float ty) {
     return lerp4(lerp4(c[0], c[1], tx), lerp4(c[2], c[3], tx), ty);
 }
@@ -145,35 +142,6 @@ __device__ __forceinline__ void kick_drift(const PathArgs &a, const Mid &m, floa
 
 constexpr float kTwelfth = 1.0f / 12.0f;
 
-// One drift-kick-drift step over a block of w cells starting at cell b0,
-// from the {G, M} values of the four columns around the mid-step position.
-__device__ __forceinline__ void block_step(const PathArgs &a, const Mid &m, const float4 (&Gc)[4],
-                                           const float4 (&Mc)[4], float w, int b0, int ncell, PathState &S, float *f) {
-    const float4 G = bilerp(Gc, m.tx, m.ty);
-    const float4 Mdx = lerp4(sub4(Mc[1], Mc[0]), sub4(Mc[3], Mc[2]), m.ty);
-    const float4 Mdy = lerp4(sub4(Mc[2], Mc[0]), sub4(Mc[3], Mc[1]), m.tx);
-    const float mvx = fmaf(m.ty, fmaf(m.tx, Mc[3].x - Mc[2].x, Mc[2].x) - fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x),
-                           fmaf(m.tx, Mc[1].x - Mc[0].x, Mc[0].x));
-    const float mvy = fmaf(m.ty, fmaf(m.tx, Mc[3].y - Mc[2].y, Mc[2].y) - fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y),
-                           fmaf(m.tx, Mc[1].y - Mc[0].y, Mc[0].y));
-    const float vx = a.sx * S.U * m.iW, vy = a.sy * S.V * m.iW;
-    float4 g = make_float4(fmaf(vy, Mdy.x, fmaf(vx, Mdx.x, G.x)), fmaf(vy, Mdy.y, fmaf(vx, Mdx.y, G.y)),
-                           fmaf(vy, Mdy.z, fmaf(vx, Mdx.z, G.z)), fmaf(vy, Mdy.w, fmaf(vx, Mdx.w, G.w)));
-    // second-order proxies from the slopes of G
-    const float Gdxx = fmaf(m.ty, (Gc[3].x - Gc[2].x) - (Gc[1].x - Gc[0].x), Gc[1].x - Gc[0].x);
-    const float Gdyx = fmaf(m.tx, (Gc[3].x - Gc[1].x) - (Gc[2].x - Gc[0].x), Gc[2].x - Gc[0].x);
-    const float Gdxy = fmaf(m.ty, (Gc[3].y - Gc[2].y) - (Gc[1].y - Gc[0].y), Gc[1].y - Gc[0].y);
-    const float Gdyy = fmaf(m.tx, (Gc[3].y - Gc[1].y) - (Gc[2].y - Gc[0].y), Gc[2].y - Gc[0].y);
-    const float iW2 = m.iW * m.iW;
-    if (m.inb) {
-        S.qx = fmaf(fmaf(Gdxx, G.x, Gdyx * G.y), iW2, S.qx);
-        S.qy = fmaf(fmaf(Gdxy, G.x, Gdyy * G.y), iW2, S.qy);
-    } else {
-        g = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    kick_drift(a, m, g, w, m.inb ? mvx : 0.f, m.inb ? mvy : 0.f, b0, ncell, S, f);
-}
-
 template <int J>
 __device__ __forceinline__ float comp(const float4 &v) {
     return J == 0 ? v.x : J == 1 ? v.y : J == 2 ? v.z : v.w;
@@ -193,7 +161,34 @@ __device__ __forceinline__ Patch patch(float c0, float c1, float c2, float c3, f
     p.dx = fmaf(ty, d23 - d01, d01);
     return p;
 }
-// patches of component J of the corner sums lo + hi and differences hi - lo
+template <int J>
+__device__ __forceinline__ Patch patch4(const float4 (&c)[4], float tx, float ty) {
+    return patch(comp<J>(c[0]), comp<J>(c[1]), comp<J>(c[2]), comp<J>(c[3]), tx, ty);
+}
+
+// One drift-kick-drift step over a block of w cells starting at cell b0,
+// from the {G, M} values of the four columns around the mid-step position.
+__device__ __forceinline__ void block_step(const PathArgs &a, const Mid &m, const float4 (&Gc)[4],
+                                           const float4 (&Mc)[4], float w, int b0, int ncell, PathState &S, float *f) {
+    const Patch g0 = patch4<0>(Gc, m.tx, m.ty), g1 = patch4<1>(Gc, m.tx, m.ty);
+    const Patch g2 = patch4<2>(Gc, m.tx, m.ty), g3 = patch4<3>(Gc, m.tx, m.ty);
+    const Patch m0 = patch4<0>(Mc, m.tx, m.ty), m1 = patch4<1>(Mc, m.tx, m.ty);
+    const Patch m2 = patch4<2>(Mc, m.tx, m.ty), m3 = patch4<3>(Mc, m.tx, m.ty);
+    // g = G + v . grad(M)
+    const float vx = a.sx * S.U * m.iW, vy = a.sy * S.V * m.iW;
+    float4 g = make_float4(fmaf(vy, m0.dy, fmaf(vx, m0.dx, g0.v)), fmaf(vy, m1.dy, fmaf(vx, m1.dx, g1.v)),
+                           fmaf(vy, m2.dy, fmaf(vx, m2.dx, g2.v)), fmaf(vy, m3.dy, fmaf(vx, m3.dx, g3.v)));
+    // second-order proxies (dGx/dx Gx + dGx/dy Gy) / W^2 and (dGy/dx Gx + dGy/dy Gy) / W^2
+    const float iW2 = m.iW * m.iW;
+    if (m.inb) {
+        S.qx = fmaf(fmaf(g0.dx, g0.v, g0.dy * g1.v), iW2, S.qx);
+        S.qy = fmaf(fmaf(g1.dx, g0.v, g1.dy * g1.v), iW2, S.qy);
+    } else {
+        g = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    kick_drift(a, m, g, w, m.inb ? m0.v : 0.f, m.inb ? m1.v : 0.f, b0, ncell, S, f);
+}
+
 template <int J>
 __device__ __forceinline__ Patch sum_patch(const float4 (&lo)[4], const float4 (&hi)[4], float tx, float ty) {
     return patch(comp<J>(lo[0]) + comp<J>(hi[0]), comp<J>(lo[1]) + comp<J>(hi[1]), comp<J>(lo[2]) + comp<J>(hi[2]),
