@@ -230,43 +230,52 @@ struct ZP {
     float4 A, B, Cc, E;  // f = A + B tx + Cc ty + E tx ty on one z-plane
 };
 
-template <bool TAU, bool VF>
-__device__ __forceinline__ void rhs_z(const TraceArgs &a, const ZP &z, float ox, float oy, const State &s, State &k,
-                                      float &gw, float &vf, float &mn, float &mx) {
+// The right-hand side at stage state s on plane z, unscaled: the derivatives
+// are r / W with r = (U, V, gx, gy, gz, -Wd) (s's velocity and the field
+// g), and 1 / W is returned in iW, so each use folds its RK4 weight into
+// the one factor c / W (one multiply) instead of scaling the five
+// derivatives first.
+template <bool VF>
+__device__ __forceinline__ void rhs_z(const TraceArgs &a, const ZP &z, float ox, float oy, const State &s, float4 &g,
+                                      float &iW, float &vf, float &mn, float &mx) {
     const float tx = fmaf(a.sx, s.qx, ox), ty = fmaf(a.sy, s.qy, oy);
     mn = fminf(mn, fminf(tx, ty));
     mx = fmaxf(mx, fmaxf(tx, ty));
-    const float4 g = f4fma(f4fma(z.E, ty, z.B), tx, f4fma(z.Cc, ty, z.A));
+    g = f4fma(f4fma(z.E, ty, z.B), tx, f4fma(z.Cc, ty, z.A));
     const float W = 1.0f + s.Wd;
-    const float iW = __builtin_amdgcn_rcpf(W);
-    k.qx = s.U * iW;
-    k.qy = s.V * iW;
-    k.U = g.x * iW;
-    k.V = g.y * iW;
-    k.Wd = g.z * iW;
-    k.tr = TAU ? -s.Wd * iW : 0.f;
-    gw = g.w;
+    iW = __builtin_amdgcn_rcpf(W);
     if (VF) vf = path_factor(s, W, iW);
+}
+
+// y + c r, r the unscaled right-hand side of stage state s with field g
+template <bool TAU>
+__device__ __forceinline__ State axpy_r(const State &y, float c, const State &s, const float4 &g) {
+    return State{fmaf(c, s.U, y.qx), fmaf(c, s.V, y.qy), fmaf(c, g.x, y.U),
+                 fmaf(c, g.y, y.V),  fmaf(c, g.z, y.Wd), TAU ? fmaf(c, -s.Wd, y.tr) : y.tr};
 }
 
 // one RK4 step of h = 1/2 over the planes z0 (start), zm (middle), z1 (end)
 template <bool TAU>
 __device__ __forceinline__ float rk4_half_z(const TraceArgs &a, const ZP &z0, const ZP &zm, const ZP &z1, float ox,
                                             float oy, State &y, float &mn, float &mx) {
-    State k, acc;
-    float p, pacc, vf = 0.f;
-    rhs_z<TAU, false>(a, z0, ox, oy, y, k, p, vf, mn, mx);
-    acc = k;
-    pacc = p;
-    rhs_z<TAU, false>(a, zm, ox, oy, axpy(y, 0.25f, k), k, p, vf, mn, mx);
-    acc = axpy(acc, 2.f, k);
-    pacc = fmaf(2.f, p, pacc);
-    rhs_z<TAU, true>(a, zm, ox, oy, axpy(y, 0.25f, k), k, p, vf, mn, mx);
-    acc = axpy(acc, 2.f, k);
-    pacc = fmaf(2.f, p, pacc);
-    rhs_z<TAU, false>(a, z1, ox, oy, axpy(y, 0.5f, k), k, p, vf, mn, mx);
-    acc = axpy(acc, 1.f, k);
-    pacc += p;
+    const State zero{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    float4 g;
+    float iW, pacc, vf = 0.f;
+    rhs_z<false>(a, z0, ox, oy, y, g, iW, vf, mn, mx);
+    State acc = axpy_r<TAU>(zero, iW, y, g);  // k1 + 2 k2 + 2 k3 + k4, k = r / W
+    pacc = g.w;
+    const State s2 = axpy_r<TAU>(y, 0.25f * iW, y, g);
+    rhs_z<false>(a, zm, ox, oy, s2, g, iW, vf, mn, mx);
+    acc = axpy_r<TAU>(acc, 2.f * iW, s2, g);
+    pacc = fmaf(2.f, g.w, pacc);
+    const State s3 = axpy_r<TAU>(y, 0.25f * iW, s2, g);
+    rhs_z<true>(a, zm, ox, oy, s3, g, iW, vf, mn, mx);
+    acc = axpy_r<TAU>(acc, 2.f * iW, s3, g);
+    pacc = fmaf(2.f, g.w, pacc);
+    const State s4 = axpy_r<TAU>(y, 0.5f * iW, s3, g);
+    rhs_z<false>(a, z1, ox, oy, s4, g, iW, vf, mn, mx);
+    acc = axpy_r<TAU>(acc, iW, s4, g);
+    pacc += g.w;
     const float h6 = 0.5f * (1.0f / 6.0f);
     y.qx = fmaf(h6, acc.qx, y.qx);
     y.qy = fmaf(h6, acc.qy, y.qy);
