@@ -311,14 +311,16 @@ __device__ __forceinline__ ZP zstep(const ZP &z, float dt, const ZP &D) {
 
 // Steps k .. K-1 for h = 1/2 (two steps per cell).  A cell is held as its
 // lower plane form and the upper-minus-lower difference (32 registers); the
-// stage planes are z0 + D/4, z0 + D/2, then stepped by D/4 from there.  When
-// a stage point of any lane leaves its cell the wave takes one uniform branch
-// in which the failing lanes redo that step with gathers.  If the redone step
-// was the cell's first, the next iteration runs the second step from the
-// cache: the lanes that redid it locate the cell they are now in, the others
-// reload their own cell (cell and planes are formed the same way on both
-// routes, so what a lane computes never depends on the other lanes of its
-// wave).  A start between z-planes (odd k) takes one gathered step first.
+// stage planes are z0 + D/4, z0 + D/2, then stepped by D/4 from there.  The
+// common case runs as a tight loop over whole cells (both steps from the
+// cache).  When a stage point of any lane leaves its cell the wave leaves
+// that loop by one uniform branch, and the failing lanes redo that step with
+// gathers.  If the redone step was the cell's first, the next iteration runs
+// the second step from the cache: the lanes that redid it locate the cell
+// they are now in, the others reload their own cell (cell and planes are
+// formed the same way on both routes, so what a lane computes never depends
+// on the other lanes of its wave).  A start between z-planes (odd k) takes
+// one gathered step first.
 template <bool TAU>
 __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
                                          double &ph) {
@@ -331,61 +333,72 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
     float ox = 0.f, oy = 0.f;  // entry point relative to the current cell
     bool inside = false;       // the cell lies in the grid (x, y)
     bool moved = false;        // this lane's last step was redone because it left its cell
+    // locate the cell of the current position
+    auto locate = [&](int &ix, int &iy) {
+        const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
+        inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
+        ix = inside ? min((int)px, a.nx - 2) : 0;
+        iy = inside ? min((int)py, a.ny - 2) : 0;
+        ox = pxb - (float)ix;
+        oy = pyb - (float)iy;
+    };
     while (k < K) {
-        const int iz = k >> 1;
-        int ix, iy;
-        if (!(k & 1) || moved) {  // a new cell, or this lane has left its cell: locate it
-            const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
-            inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
-            ix = inside ? min((int)px, a.nx - 2) : 0;
-            iy = inside ? min((int)py, a.ny - 2) : 0;
-            ox = pxb - (float)ix;
-            oy = pyb - (float)iy;
-        } else {  // another lane's first step was redone: this lane's cell (|rounding of ox| << 1/2)
-            ix = (int)rintf(pxb - ox);
-            iy = (int)rintf(pyb - oy);
-        }
-        moved = false;
-        ZP z0, D;
-        load_planes(a, ix, iy, iz, z0, D);
-        float mn = inside ? 0.f : -1.f, mx = 0.f;
-        State yn = ys;
+        State yn;
         float dph;
         bool ok;
-        ZP zh;
-        if (!(k & 1)) {
-            const ZP zq = zstep(z0, 0.25f, D);
-            zh = zstep(z0, 0.5f, D);
-            dph = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx);
-            ok = mn >= 0.f && mx <= 1.f;
-            if (__ballot(!ok) != 0) goto redo;
-            ys = yn;
-            ph += (double)dph;
-            ++k;
-            good = good && (ys.Wd > -1.0f);
-            yn = ys;
-        } else {
-            zh = zstep(z0, 0.5f, D);
-        }
-        {
+        if (k & 1) {  // the second step of a cell whose first step was redone
+            int ix, iy;
+            if (moved) {
+                locate(ix, iy);
+            } else {  // this lane's own cell (|rounding of ox| << 1/2)
+                ix = (int)rintf(pxb - ox);
+                iy = (int)rintf(pyb - oy);
+            }
+            ZP z0, D;
+            load_planes(a, ix, iy, k >> 1, z0, D);
+            float mn = inside ? 0.f : -1.f, mx = 0.f;
+            const ZP zh = zstep(z0, 0.5f, D);
             const ZP z3 = zstep(zh, 0.25f, D);
             const ZP z1 = zstep(z3, 0.25f, D);
+            yn = ys;
             dph = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
+            ok = mn >= 0.f && mx <= 1.f;
+        } else {  // whole cells from the cache
+            for (;;) {
+                int ix, iy;
+                locate(ix, iy);
+                ZP z0, D;
+                load_planes(a, ix, iy, k >> 1, z0, D);
+                float mn = inside ? 0.f : -1.f, mx = 0.f;
+                const ZP zq = zstep(z0, 0.25f, D);
+                const ZP zh = zstep(z0, 0.5f, D);
+                yn = ys;
+                dph = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx);
+                ok = mn >= 0.f && mx <= 1.f;
+                if (__ballot(!ok) != 0) break;  // redo the cell's first step below
+                ys = yn;
+                ph += (double)dph;
+                ++k;
+                good = good && (ys.Wd > -1.0f);
+                const ZP z3 = zstep(zh, 0.25f, D);
+                const ZP z1 = zstep(z3, 0.25f, D);
+                yn = ys;
+                dph = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
+                ok = mn >= 0.f && mx <= 1.f;
+                if (__ballot(!ok) != 0) break;  // redo the cell's second step below
+                ys = yn;
+                ph += (double)dph;
+                ++k;
+                good = good && (ys.Wd > -1.0f);
+                if (k >= K) return good;
+            }
         }
-        ok = mn >= 0.f && mx <= 1.f;
-        if (__ballot(!ok) == 0) {
-            ys = yn;
-            ph += (double)dph;
-            ++k;
-            good = good && (ys.Wd > -1.0f);
-            continue;
-        }
-    redo:  // the cell is dead here; the lanes that left it redo step k with gathers
+        // step k: the lanes that left their cell redo it with gathers
         if (!ok) {
             yn = ys;
             dph = rk4_step(a, pxb, pyb, (float)k * 0.5f, 0.5f, yn);
-            moved = true;
         }
+        moved = !ok;
         ys = yn;
         ph += (double)dph;
         ++k;
