@@ -189,41 +189,60 @@ __device__ __forceinline__ void block_step(const PathArgs &a, const Mid &m, cons
     kick_drift(a, m, g, w, m.inb ? m0.v : 0.f, m.inb ? m1.v : 0.f, b0, ncell, S, f);
 }
 
-template <int J>
-__device__ __forceinline__ Patch sum_patch(const float4 (&lo)[4], const float4 (&hi)[4], float tx, float ty) {
-    return patch(comp<J>(lo[0]) + comp<J>(hi[0]), comp<J>(lo[1]) + comp<J>(hi[1]), comp<J>(lo[2]) + comp<J>(hi[2]),
-                 comp<J>(lo[3]) + comp<J>(hi[3]), tx, ty);
+// One node plane over a column square as its bilinear form
+// f = A + B tx + C ty + E tx ty (corners c0 (0, 0), c1 (1, 0), c2 (0, 1),
+// c3 (1, 1): B = c1 - c0, C = c2 - c0, E = c3 - c2 - B).
+struct PlaneForm {
+    float4 A, B, C, E;
+};
+__device__ __forceinline__ PlaneForm plane_form(const float4 *p, int nx) {
+    const float4 c0 = p[0], c1 = p[1], c2 = p[nx], c3 = p[nx + 1];
+    PlaneForm f;
+    f.A = c0;
+    f.B = make_float4(c1.x - c0.x, c1.y - c0.y, c1.z - c0.z, c1.w - c0.w);
+    f.C = make_float4(c2.x - c0.x, c2.y - c0.y, c2.z - c0.z, c2.w - c0.w);
+    f.E = make_float4(c3.x - c2.x - f.B.x, c3.y - c2.y - f.B.y, c3.z - c2.z - f.B.z, c3.w - c2.w - f.B.w);
+    return f;
 }
+// value and x / y slopes of component J of a plane form at (tx, ty)
 template <int J>
-__device__ __forceinline__ Patch dif_patch(const float4 (&lo)[4], const float4 (&hi)[4], float tx, float ty) {
-    return patch(comp<J>(hi[0]) - comp<J>(lo[0]), comp<J>(hi[1]) - comp<J>(lo[1]), comp<J>(hi[2]) - comp<J>(lo[2]),
-                 comp<J>(hi[3]) - comp<J>(lo[3]), tx, ty);
+__device__ __forceinline__ Patch form_patch(const PlaneForm &f, float tx, float ty) {
+    const float a = fmaf(tx, comp<J>(f.B), comp<J>(f.A));
+    Patch p;
+    p.dy = fmaf(tx, comp<J>(f.E), comp<J>(f.C));
+    p.v = fmaf(ty, p.dy, a);
+    p.dx = fmaf(ty, comp<J>(f.E), comp<J>(f.B));
+    return p;
 }
 
 // One step over the single cell c from its node planes lo (c) and hi
 // (c + 1) at the four columns around the mid-step position: the block step
-// with G = (lo + hi) / 2 and M = (hi - lo) / 12, both interpolated
-// bilinearly (so the sums and differences are interpolated once, and the
-// factors 1/2 and 1/12 applied to the interpolated values).
-__device__ __forceinline__ void node_step(const PathArgs &a, const Mid &m, const float4 (&lo)[4],
-                                          const float4 (&hi)[4], int c, int ncell, PathState &S, float *f) {
-    const Patch s0 = sum_patch<0>(lo, hi, m.tx, m.ty), s1 = sum_patch<1>(lo, hi, m.tx, m.ty);
-    const Patch s2 = sum_patch<2>(lo, hi, m.tx, m.ty), s3 = sum_patch<3>(lo, hi, m.tx, m.ty);
-    const Patch d0 = dif_patch<0>(lo, hi, m.tx, m.ty), d1 = dif_patch<1>(lo, hi, m.tx, m.ty);
-    const Patch d2 = dif_patch<2>(lo, hi, m.tx, m.ty), d3 = dif_patch<3>(lo, hi, m.tx, m.ty);
-    // g = G + v . grad(M) = (s + (v / 6) . grad(d)) / 2
+// with G = (lo + hi) / 2 and M = (hi - lo) / 12 interpolated bilinearly
+// (each plane interpolated once, the factors 1/2 and 1/12 applied to the
+// interpolated sums and differences).
+__device__ __forceinline__ void node_step(const PathArgs &a, const Mid &m, const PlaneForm &lo, const PlaneForm &hi,
+                                          int c, int ncell, PathState &S, float *f) {
+    const Patch l0 = form_patch<0>(lo, m.tx, m.ty), l1 = form_patch<1>(lo, m.tx, m.ty);
+    const Patch l2 = form_patch<2>(lo, m.tx, m.ty), l3 = form_patch<3>(lo, m.tx, m.ty);
+    const Patch h0 = form_patch<0>(hi, m.tx, m.ty), h1 = form_patch<1>(hi, m.tx, m.ty);
+    const Patch h2 = form_patch<2>(hi, m.tx, m.ty), h3 = form_patch<3>(hi, m.tx, m.ty);
+    // g = G + v . grad(M) = (s + (v / 6) . grad(d)) / 2, s = lo + hi, d = hi - lo
     const float vx6 = a.sx * S.U * m.iW * (1.0f / 6.0f), vy6 = a.sy * S.V * m.iW * (1.0f / 6.0f);
-    float4 g = make_float4(0.5f * fmaf(vy6, d0.dy, fmaf(vx6, d0.dx, s0.v)), 0.5f * fmaf(vy6, d1.dy, fmaf(vx6, d1.dx, s1.v)),
-                           0.5f * fmaf(vy6, d2.dy, fmaf(vx6, d2.dx, s2.v)), 0.5f * fmaf(vy6, d3.dy, fmaf(vx6, d3.dx, s3.v)));
+    float4 g = make_float4(0.5f * fmaf(vy6, h0.dy - l0.dy, fmaf(vx6, h0.dx - l0.dx, l0.v + h0.v)),
+                           0.5f * fmaf(vy6, h1.dy - l1.dy, fmaf(vx6, h1.dx - l1.dx, l1.v + h1.v)),
+                           0.5f * fmaf(vy6, h2.dy - l2.dy, fmaf(vx6, h2.dx - l2.dx, l2.v + h2.v)),
+                           0.5f * fmaf(vy6, h3.dy - l3.dy, fmaf(vx6, h3.dx - l3.dx, l3.v + h3.v)));
     // second-order proxies (dGx/dx Gx + dGx/dy Gy) / W^2 with G = s / 2
     const float iW2q = 0.25f * m.iW * m.iW;
+    const float s0v = l0.v + h0.v, s1v = l1.v + h1.v;
     if (m.inb) {
-        S.qx = fmaf(fmaf(s0.dx, s0.v, s0.dy * s1.v), iW2q, S.qx);
-        S.qy = fmaf(fmaf(s1.dx, s0.v, s1.dy * s1.v), iW2q, S.qy);
+        S.qx = fmaf(fmaf(l0.dx + h0.dx, s0v, (l0.dy + h0.dy) * s1v), iW2q, S.qx);
+        S.qy = fmaf(fmaf(l1.dx + h1.dx, s0v, (l1.dy + h1.dy) * s1v), iW2q, S.qy);
     } else {
         g = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    kick_drift(a, m, g, 1.0f, m.inb ? d0.v * kTwelfth : 0.f, m.inb ? d1.v * kTwelfth : 0.f, c, ncell, S, f);
+    kick_drift(a, m, g, 1.0f, m.inb ? (h0.v - l0.v) * kTwelfth : 0.f, m.inb ? (h1.v - l1.v) * kTwelfth : 0.f, c,
+               ncell, S, f);
 }
 
 __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
@@ -283,26 +302,24 @@ __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
         // node planes, G = (lo + hi) / 2 and M = (hi - lo) / 12 (what
         // block_kernel stores for k = 1), so each step reads the node field:
         // the upper plane's four corners, and the lower plane's only when the
-        // ray has changed column (else it is the last step's upper plane).
-        // Half the bytes per step of the block fields, no block fields to
-        // build; loading the next upper plane one step ahead was slower
-        // (more registers, fewer waves).
+        // ray has changed column (else it is the last step's upper plane, kept
+        // as its bilinear form).  Half the bytes per step of the block fields,
+        // no block fields to build; loading the next upper plane one step
+        // ahead was slower (more registers, fewer waves).
         const int64_t plane = (int64_t)a.nx * a.ny;
         int qx = -1, qy = -1;
-        float4 lo[4];
+        PlaneForm lo;
         for (int c = bstart; c < ncell; ++c) {
             const Mid m = half_drift(a, S, pxb, pyb, 1.0f);
             const float4 *p = a.field + ((int64_t)(c + 1) * a.ny + m.iy) * a.nx + m.ix;
-            float4 hi[4];
-            load_corners(p, a.nx, hi);
+            const PlaneForm hi = plane_form(p, a.nx);
             if (m.ix != qx || m.iy != qy) {
-                load_corners(p - plane, a.nx, lo);
+                lo = plane_form(p - plane, a.nx);
                 qx = m.ix;
                 qy = m.iy;
             }
             node_step(a, m, lo, hi, c, ncell, S, f);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) lo[q] = hi[q];
+            lo = hi;
         }
     } else {
         // z-blocks: a column's blocks are consecutive in memory, so a wave's
