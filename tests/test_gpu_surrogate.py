@@ -251,6 +251,10 @@ def test_surrogate_gain_and_fallback(lib):
     assert 0 < int(fl.sum()) < p.s0.shape[1]
     np.testing.assert_array_equal(rf3[:, fl].cpu().numpy(), p.rf[:, fl].cpu().numpy())
     np.testing.assert_array_equal(ph3[fl].cpu().numpy(), p.phase[fl].cpu().numpy())
+    # the rays cut into batches on two streams: every ray's result is the same
+    rf4, ph4, flag4 = sur.trace(p.field4, p.grid, p.s0, t_pos=1.0, t_ang=t_ang, chunks=4)
+    assert torch.equal(flag4, flag3)
+    assert torch.equal(rf4.nan_to_num(7.0), rf3.nan_to_num(7.0)) and torch.equal(ph4.nan_to_num(7.0), ph3.nan_to_num(7.0))
     # fallback forced from the host: every ray re-traced -> exactly the tracer
     rf2, ph2, flag2 = sur.trace(p.field4, p.grid, p.s0, t_pos=0.0, t_ang=0.0)
     assert int(flag2.sum()) == p.s0.shape[1]

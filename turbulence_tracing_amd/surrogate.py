@@ -403,7 +403,7 @@ class Surrogate:
             self._streams = (torch.cuda.Stream(device=self.device), torch.cuda.Stream(device=self.device))
         return self._streams
 
-    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, blocks=None, timings=None, chunks=4):
+    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, blocks=None, timings=None, chunks=1):
         """Surrogate + fallback trace of rays s0 (6 x n, device): the block
         path estimate, the network's correction, and the full RK4 tracer on
         the rays the network flags (predicted position error above t_pos
@@ -411,11 +411,13 @@ class Surrogate:
         estimate).  Returns rf (4 x n), phase (n) and the flag per ray
         (1 = re-traced).
 
-        The rays are cut into ``chunks`` batches: the path estimate of batch
-        i + 1 runs on one stream while the network and the correction of
-        batch i run on another, so the VALU-bound estimate and the
-        MFMA-bound network share the CUs.  ``timings`` (a dict) receives the
-        total and fallback ms (and per-stage ms when chunks == 1)."""
+        With ``chunks`` > 1 the rays are cut into that many batches: the path
+        estimate of batch i + 1 runs on one stream while the network and the
+        correction of batch i run on another.  The two kernels then compete
+        for the same CUs: on MI355X this measured the same as one batch at
+        512^3 (k = 16) and ~1 ms slower per 1e7 rays at 1024^3 (k = 1), so
+        the default is one batch.  ``timings`` (a dict) receives the total
+        and fallback ms (and per-stage ms when chunks == 1)."""
         n = s0.shape[1]
         dev = self.device
         main = torch.cuda.current_stream()
