@@ -74,3 +74,53 @@ def test_waves_surrogate_all_flagged_is_plain(lib):
         np.testing.assert_array_equal(a, c)
         np.testing.assert_array_equal(b, d)
     np.testing.assert_array_equal(H, run.p.H_shadow.cpu().numpy())
+
+
+def test_large_volume_full_size_subset_matches_oracle(lib):
+    """The large workload at its real shape (1024^3 field, L = 20 mm, 15%
+    RMS, waves of 1e7 rays from the 8 mm beam, 4096^2 detectors), two waves:
+    256 consecutive rays of the last wave's entry-column order (one compact
+    patch of columns) match the FP64 integrator on the node field cropped
+    around their paths, every ray of the wave leaves finite, and the
+    shadowgraph holds nearly every ray of both waves."""
+    from oracle import tracer as OT
+    from turbulence_tracing_amd.waves import WaveRun, large_volume_config
+    cfg = large_volume_config()
+    assert (cfg.n, cfg.pix) == (1024, 4096)
+    run = WaveRun(cfg, 20_000_000, 10_000_000)
+    run.field(7)
+    run.run(100)
+    torch.cuda.synchronize()
+    p = run.p
+    assert bool(torch.isfinite(p.rf).all()) and bool(torch.isfinite(p.phase).all())
+    assert int(p.H_shadow.sum()) > 0.99 * 20_000_000
+    mid = run.wave // 2
+    s0 = p.s0[:, mid:mid + 256].cpu().numpy()
+    rf = p.rf[:, mid:mid + 256].cpu().numpy()
+    ph = p.phase[mid:mid + 256].cpu().numpy()
+    gr = p.grid
+    ix = (s0[0] - gr.x0) / gr.dx
+    iy = (s0[1] - gr.y0) / gr.dy
+    m = 24  # cells: RMS deflection 3.6 mrad over 1023 cells is ~4 cells
+    x_lo, x_hi = max(int(ix.min()) - m, 0), min(int(ix.max()) + m + 1, gr.nx - 1)
+    y_lo, y_hi = max(int(iy.min()) - m, 0), min(int(iy.max()) + m + 1, gr.ny - 1)
+    f4 = p.field4[:, y_lo:y_hi + 1, x_lo:x_hi + 1].double().cpu().numpy()
+    dz = gr.dz
+    k0 = 2 * np.pi / cfg.lwl
+    g = OT.NodeGrid.__new__(OT.NodeGrid)
+    g.shape = (x_hi - x_lo + 1, y_hi - y_lo + 1, gr.nz)
+    g.origin = (gr.x0 + x_lo * gr.dx, gr.y0 + y_lo * gr.dy, gr.z0)
+    g.spacing = (gr.dx, gr.dy, gr.dz)
+    g.fields = tuple(np.ascontiguousarray(f4[..., ch].transpose(2, 1, 0)) / (dz if ch < 3 else k0 * dz)
+                     for ch in range(4))
+    g.k0 = k0
+    g.extent = cfg.L / 2
+    rf_o, ph_o, _, st = OT.zmarch_rk4(g, s0)
+    assert (st == 0).all()
+    px, py = (rf_o[0] - g.origin[0]) / gr.dx, (rf_o[2] - g.origin[1]) / gr.dy
+    assert (px > 2).all() and (px < g.shape[0] - 3).all() and (py > 2).all() and (py < g.shape[1] - 3).all()
+    pos_err = np.abs(rf[[0, 2]] - rf_o[[0, 2]]).max() / gr.dx
+    defl = np.sqrt(np.mean(rf_o[[1, 3]] ** 2))
+    assert pos_err <= 1e-3, pos_err
+    assert np.abs(rf[[1, 3]] - rf_o[[1, 3]]).max() <= 1e-3 * defl
+    assert np.abs(ph - ph_o).max() <= 1e-5 * np.abs(ph_o).max()
