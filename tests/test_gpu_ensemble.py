@@ -114,3 +114,28 @@ def test_autotune_choice_and_cache(lib, tmp_path):
     ens.step()
     for a, b in zip(tuned, snapshot(ens)):
         assert_same(a, b)
+
+
+def test_full_ensemble_batch_equals_per_member(lib):
+    """The ensemble workload at its real size (the seeded draw of 48 members,
+    128-384^3, 2e5-2e6 rays each, ~51 M rays resident): the one persistent
+    batched launch gives every member's exit rays and phases bit for bit as
+    one tt_trace_rk4 per member, and nearly every ray leaves finite (a ray
+    that turns back is NaN by design)."""
+    from turbulence_tracing_amd.ensemble import Ensemble, draw_members
+    members = draw_members(48, 0)
+    assert len(members) == 48 and sum(m.n_rays for m in members) > 40_000_000
+    ens = Ensemble(members, pix=512)
+    ens.generate()
+    ens.order()
+    ens.trace()
+    torch.cuda.synchronize()
+    batched = [(p.rf.clone(), p.phase.clone()) for p in ens.pipes]
+    for (rf, ph), p in zip(batched, ens.pipes):
+        p.rf.fill_(7.0)
+        p.phase.fill_(7.0)
+        p.trace()
+        torch.cuda.synchronize()
+        assert torch.equal(p.rf.nan_to_num(3.0), rf.nan_to_num(3.0))
+        assert torch.equal(p.phase.nan_to_num(3.0), ph.nan_to_num(3.0))
+        assert float(torch.isfinite(ph).double().mean()) > 0.999
