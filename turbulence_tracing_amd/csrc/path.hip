@@ -245,7 +245,11 @@ __device__ __forceinline__ void node_step(const PathArgs &a, const Mid &m, const
                ncell, S, f);
 }
 
-__global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
+// NODE: the k = 1 steps from the node field (no block fields); it runs at
+// five waves per SIMD (a few registers spill outside the step loop), which
+// hides more of each step's load latency; the block-field form keeps four.
+template <bool NODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NODE ? 5 : 1))) void path_kernel(PathArgs a) {
     const int64_t t = (int64_t)tt::xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     const int64_t r = a.order ? (int64_t)a.order[t] : t;
@@ -297,7 +301,7 @@ __global__ __launch_bounds__(256) void path_kernel(PathArgs a) {
         g = m.inb ? make_float4(w * g.x, w * g.y, w * g.z, w * g.w) : make_float4(0.f, 0.f, 0.f, 0.f);
         kick_drift(a, m, g, w, 0.f, 0.f, c, ncell, S, f);
     }
-    if (!a.GM) {
+    if (NODE) {
         // k = 1 without block fields: a cell's G and M follow from its two
         // node planes, G = (lo + hi) / 2 and M = (hi - lo) / 12 (what
         // block_kernel stores for k = 1), so each step reads the node field:
@@ -509,7 +513,12 @@ extern "C" int tt_path_features(const float *field4, const float *GM4, int32_t k
     a.ldf = ldf;
     a.base = base;
     a.ldb = ldb;
-    hipLaunchKernelGGL(path_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream), a);
+    if (a.GM)
+        hipLaunchKernelGGL(path_kernel<false>, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0,
+                           tt::as_stream(stream), a);
+    else
+        hipLaunchKernelGGL(path_kernel<true>, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0,
+                           tt::as_stream(stream), a);
     TT_LAUNCH_CHECK("tt_path_features launch");
     return TT_OK;
 }
