@@ -380,43 +380,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NODE ? 5 : 
     gf[15 * L1] = (float)L;
 }
 
-// Block integrals: thread per node column, marching down z once; for each
-// block of cells [b0, b1): G = sum_c (f[c] + f[c+1]) / 2 and
+// Block integrals: for each block of cells [b0, b1) of a node column,
+// G = sum_c (f[c] + f[c+1]) / 2 and
 // M = sum_c (c + 1/2 - zmid) (f[c] + f[c+1]) / 2 + (f[b1] - f[b0]) / 12,
-// stored as GM[col][b] = {G, M}.
-__global__ void block_kernel(const float4 *__restrict__ f, float4 *__restrict__ GM, int64_t plane, int ncell, int k,
-                             int nb) {
+// stored as GM[col][b] = {G, M}.  Thread per (node column, block): blockIdx.y
+// is the block, so the grid has plane x nb threads and every thread streams
+// only its block's k + 1 planes.
+__global__ __launch_bounds__(256) void block_kernel(const float4 *__restrict__ f, float4 *__restrict__ GM,
+                                                    int64_t plane, int ncell, int k, int nb) {
     const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= plane) return;
-    float4 *out = GM + col * 2 * nb;
-    float4 lo = f[col];
-    for (int bb = 0; bb < nb; ++bb) {
-        const int b0 = bb * k, b1 = min(ncell, b0 + k);
-        const float zm = 0.5f * (float)(b0 + b1);
-        const float4 first = lo;
-        float4 g = make_float4(0.f, 0.f, 0.f, 0.f), m = g;
-        for (int c = b0; c < b1; ++c) {
-            const float4 hi = f[(int64_t)(c + 1) * plane + col];
-            const float4 mid = make_float4(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z),
-                                           0.5f * (lo.w + hi.w));
-            const float zc = (float)c + 0.5f - zm;
-            g.x += mid.x;
-            g.y += mid.y;
-            g.z += mid.z;
-            g.w += mid.w;
-            m.x = fmaf(zc, mid.x, m.x);
-            m.y = fmaf(zc, mid.y, m.y);
-            m.z = fmaf(zc, mid.z, m.z);
-            m.w = fmaf(zc, mid.w, m.w);
-            lo = hi;
-        }
-        m.x = fmaf(lo.x - first.x, kTwelfth, m.x);
-        m.y = fmaf(lo.y - first.y, kTwelfth, m.y);
-        m.z = fmaf(lo.z - first.z, kTwelfth, m.z);
-        m.w = fmaf(lo.w - first.w, kTwelfth, m.w);
-        out[2 * bb] = g;
-        out[2 * bb + 1] = m;
+    const int bb = blockIdx.y;
+    const int b0 = bb * k, b1 = min(ncell, b0 + k);
+    const float zm = 0.5f * (float)(b0 + b1);
+    float4 lo = f[(int64_t)b0 * plane + col];
+    const float4 first = lo;
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f), m = g;
+    for (int c = b0; c < b1; ++c) {
+        const float4 hi = f[(int64_t)(c + 1) * plane + col];
+        const float4 mid = make_float4(0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z),
+                                       0.5f * (lo.w + hi.w));
+        const float zc = (float)c + 0.5f - zm;
+        g.x += mid.x;
+        g.y += mid.y;
+        g.z += mid.z;
+        g.w += mid.w;
+        m.x = fmaf(zc, mid.x, m.x);
+        m.y = fmaf(zc, mid.y, m.y);
+        m.z = fmaf(zc, mid.z, m.z);
+        m.w = fmaf(zc, mid.w, m.w);
+        lo = hi;
     }
+    m.x = fmaf(lo.x - first.x, kTwelfth, m.x);
+    m.y = fmaf(lo.y - first.y, kTwelfth, m.y);
+    m.z = fmaf(lo.z - first.z, kTwelfth, m.z);
+    m.w = fmaf(lo.w - first.w, kTwelfth, m.w);
+    float4 *out = GM + (col * nb + bb) * 2;
+    out[0] = g;
+    out[1] = m;
 }
 
 // corrected = base + Y[:, :5] * scale; flag when a predicted error bound
@@ -472,7 +473,8 @@ extern "C" int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t
     const int ncell = grid->nz - 1, nb = (ncell + k - 1) / k;
     TT_REQUIRE(nb <= 65535, "tt_field_blocks: too many blocks");
     const int64_t plane = (int64_t)grid->nx * grid->ny;
-    hipLaunchKernelGGL(block_kernel, dim3((unsigned)tt::ceil_div(plane, 256)), dim3(256), 0, tt::as_stream(stream),
+    hipLaunchKernelGGL(block_kernel, dim3((unsigned)tt::ceil_div(plane, 256), (unsigned)nb), dim3(256), 0,
+                       tt::as_stream(stream),
                        reinterpret_cast<const float4 *>(field4), reinterpret_cast<float4 *>(GM4), plane, ncell, (int)k,
                        nb);
     TT_LAUNCH_CHECK("tt_field_blocks launch");
