@@ -18,6 +18,9 @@
  *    [x, theta, y, phi] in m and rad (mm after the optics, like the
  *    reference's m_to_mm).
  *  - Grid fields are stored x-fastest, z-slowest: index (iz*ny + iy)*nx + ix.
+ *  - A ray-parallel call with n = 0 (an empty ray set) checks its sizes and
+ *    host structs only, touches no device memory and returns TT_OK, so its
+ *    device pointers may be NULL (an empty torch tensor has none).
  *  - The caller owns every device buffer.  The library allocates only inside
  *    an FFT plan (its twiddle table), released by tt_fft_plan_destroy.
  */

@@ -217,6 +217,27 @@ def test_path_k1_node_field_matches_block_fields(lib, inside):
                              None, f_node.data_ptr(), f_node.stride(1), b_node.data_ptr(), b_node.shape[1], None)
 
 
+def test_empty_ray_set_through_the_surrogate(lib):
+    """An empty ray set goes through the estimate, the network, the
+    correction and the surrogate trace as empty results."""
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    from turbulence_tracing_amd.surrogate import MLP, Norm, Surrogate, path_estimate
+    x, ne = small_cube(n=17)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne)
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(16, 2e-3, 50e-6, seed=3)[:, :0]
+    for k in (1, 4):
+        base, feat = path_estimate(cube.field4, cube.grid, s0, k=k)
+        assert tuple(base.shape) == (5, 0) and tuple(feat.shape) == (0, 48)
+    sur = Surrogate(Norm(np.zeros(48, np.float32), np.ones(48, np.float32), 1e-6, 1e-6, 1e-3), mlp=MLP(seed=2), k=4)
+    Y = sur.predict(feat)
+    assert tuple(Y.shape) == (0, 8)
+    rf, ph, flag = sur.trace(cube.field4, cube.grid, s0)
+    torch.cuda.synchronize()
+    assert tuple(rf.shape) == (4, 0) and ph.numel() == 0 and flag.numel() == 0
+
+
 def test_surrogate_gain_and_fallback(lib):
     """Trained on rays the tracer generates through 32 seeded realisations,
     the network corrects the block path estimate on a held-out realisation:

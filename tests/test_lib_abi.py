@@ -92,3 +92,21 @@ def test_struct_layouts_match_c(tmp_path):
         assert got[(cname, "size")] == _lib.C.sizeof(st), cname
         for f in st._fields_:
             assert got[(cname, f[0])] == getattr(st, f[0]).offset, (cname, f[0])
+
+
+def test_empty_ray_sets_need_no_device_memory():
+    """n = 0 on every ray-parallel entry point returns success with NULL
+    device pointers and launches nothing (so it runs without a GPU)."""
+    lib = _lib.load()
+    g = _lib.Grid(4, 4, 4, 0, 0, 0, 1, 1, 1)
+    opts = _lib.TraceOpts(0.5, 1.0)
+    lib.tt_trace_rk4(None, _lib.C.byref(g), _lib.C.byref(opts), None, 0, 0, None, None, 0, None, None, 0, None, None)
+    lib.tt_sample_field(None, _lib.C.byref(g), None, 0, 0, None, 0, None)
+    lib.tt_beam_sample(0, 1e-3, 1e-5, 1.0, 2, 1, 0, None, 0, None)
+    lib.tt_ray_sort(None, 0, 0, _lib.C.byref(g), 0, None, None, None)
+    lib.tt_ray_sort_rays(None, 0, 0, _lib.C.byref(g), 0, None, None, None, None, 0, None)
+    lib.tt_path_features(None, None, 4, _lib.C.byref(g), None, 0, 0, None, None, 0, None, 0, None)
+    lib.tt_mlp_infer(None, 48, 1, 0, 48, None, None, None, None, None, 8, 8, None)
+    lib.tt_surrogate_apply(None, 0, None, 8, 0, None, None, None, 0, None, None, None, None, 0, None)
+    t = _lib.Train()
+    lib.tt_optics_apply(_lib.C.byref(t), None, 0, 0, None, 0, None)

@@ -251,8 +251,9 @@ extern "C" int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_gri
 extern "C" int tt_ray_sort_rays(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
                                 void *scratch, int32_t *order, double *staging, double *out, int64_t ldo,
                                 void *stream) {
-    TT_REQUIRE(staging && out && ((uintptr_t)staging & 15) == 0, "tt_ray_sort_rays: staging must be 16-byte aligned");
     TT_REQUIRE(ldo >= n, "tt_ray_sort_rays: ldo < n");
-    TT_REQUIRE(out != s0, "tt_ray_sort_rays: out must not alias s0");
+    TT_REQUIRE(n == 0 || (staging && out && ((uintptr_t)staging & 15) == 0),
+               "tt_ray_sort_rays: staging must be 16-byte aligned");
+    TT_REQUIRE(n == 0 || out != s0, "tt_ray_sort_rays: out must not alias s0");
     return sort_impl(s0, ld, n, grid, brick_shift, scratch, order, staging, out, ldo, stream);
 }

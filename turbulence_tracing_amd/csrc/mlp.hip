@@ -522,14 +522,14 @@ extern "C" int tt_standardize_bf16(const float *x, int64_t srow, int64_t scol, i
 extern "C" int tt_mlp_infer(const float *feat, int64_t srow, int64_t scol, int64_t n, int32_t nfeat, const float *mu,
                             const float *inv_sd, const void *w_packed, const float *bias_packed, float *Y, int64_t ldy,
                             int32_t nout, void *stream) {
+    TT_REQUIRE(n >= 0, "tt_mlp_infer: n < 0");
+    if (n == 0) return TT_OK;  // an empty ray set: nothing is read or written
     TT_REQUIRE(feat && mu && inv_sd && w_packed && bias_packed && Y, "tt_mlp_infer: null pointer");
     TT_REQUIRE(nfeat > 0 && nfeat <= 64, "tt_mlp_infer: need 0 < nfeat <= 64");
     TT_REQUIRE(srow >= 1 && scol >= 1, "tt_mlp_infer: strides must be >= 1");
     TT_REQUIRE(nout > 0 && nout <= 16 && ldy >= nout, "tt_mlp_infer: need 0 < nout <= 16 and ldy >= nout");
     TT_REQUIRE(((uintptr_t)w_packed & 15) == 0 && ((uintptr_t)bias_packed & 15) == 0,
                "tt_mlp_infer: packed weights and biases must be 16-byte aligned");
-    TT_REQUIRE(n >= 0, "tt_mlp_infer: n < 0");
-    if (n == 0) return TT_OK;
     InferArgs a;
     a.feat = feat;
     a.srow = srow;

@@ -454,11 +454,11 @@ extern "C" int tt_surrogate_apply(const double *base, int64_t ldb, const float *
                                   const float *scales, const double *thresholds, double *rf, int64_t ldr,
                                   double *phase, int32_t *flag, int32_t *list, int32_t *nflag, int64_t first,
                                   void *stream) {
-    TT_REQUIRE(base && Y && scales && thresholds && rf, "tt_surrogate_apply: null pointer");
     TT_REQUIRE(n >= 0 && ldb >= n && ldr >= n && ldy >= 7, "tt_surrogate_apply: bad sizes");
-    TT_REQUIRE(!list || nflag, "tt_surrogate_apply: list needs nflag");
     TT_REQUIRE(first >= 0 && first + n <= 0x7fffffff, "tt_surrogate_apply: ray indices must fit int32");
-    if (n == 0) return TT_OK;
+    if (n == 0) return TT_OK;  // an empty ray set: nothing is read or written
+    TT_REQUIRE(base && Y && scales && thresholds && rf, "tt_surrogate_apply: null pointer");
+    TT_REQUIRE(!list || nflag, "tt_surrogate_apply: list needs nflag");
     hipLaunchKernelGGL(apply_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream), base,
                        ldb, Y, ldy, n, scales[0], scales[1], scales[2], scales[3], scales[4], thresholds[0],
                        thresholds[1], rf, ldr, phase, flag, list, nflag, first);
@@ -484,13 +484,14 @@ extern "C" int tt_field_blocks(const float *field4, const tt_grid *grid, int32_t
 extern "C" int tt_path_features(const float *field4, const float *GM4, int32_t k, const tt_grid *grid, const double *s0,
                                 int64_t ld, int64_t n, const int32_t *order, float *feat, int64_t ldf, double *base,
                                 int64_t ldb, void *stream) {
-    TT_REQUIRE(field4 && grid && s0 && feat && base, "tt_path_features: null pointer");
-    TT_REQUIRE(GM4 || k == 1, "tt_path_features: block fields GM4 are required for k > 1");
+    TT_REQUIRE(grid, "tt_path_features: null grid");
     TT_REQUIRE(n >= 0 && ld >= n && ldb >= n && ldf >= n, "tt_path_features: bad sizes");
     TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "tt_path_features: grid needs >= 2 nodes per axis");
     TT_REQUIRE(grid->dx > 0 && grid->dy > 0 && grid->dz > 0, "tt_path_features: spacing must be > 0");
     TT_REQUIRE(k >= 1 && k <= 1024, "tt_path_features: block size k must be in [1, 1024]");
-    if (n == 0) return TT_OK;
+    if (n == 0) return TT_OK;  // an empty ray set: nothing is read or written
+    TT_REQUIRE(field4 && s0 && feat && base, "tt_path_features: null pointer");
+    TT_REQUIRE(GM4 || k == 1, "tt_path_features: block fields GM4 are required for k > 1");
     PathArgs a;
     a.field = reinterpret_cast<const float4 *>(field4);
     a.GM = reinterpret_cast<const float4 *>(GM4);

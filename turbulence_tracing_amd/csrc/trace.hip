@@ -634,9 +634,10 @@ int fill_grid_args(const tt_grid *grid, TraceArgs &a) {
 
 extern "C" int tt_sample_field(const float *field4, const tt_grid *grid, const double *pts, int64_t ld, int64_t n,
                                double *out, int64_t ld_out, void *stream) {
-    TT_REQUIRE(field4 && grid && pts && out, "tt_sample_field: null pointer");
+    TT_REQUIRE(grid, "tt_sample_field: null grid");
     TT_REQUIRE(n >= 0 && ld >= n && ld_out >= n, "tt_sample_field: bad sizes");
-    if (n == 0) return TT_OK;
+    if (n == 0) return TT_OK;  // an empty point set: nothing is read or written
+    TT_REQUIRE(field4 && pts && out, "tt_sample_field: null pointer");
     TraceArgs a{};
     if (int rc = fill_grid_args(grid, a)) return rc;
     a.field = reinterpret_cast<const float4 *>(field4);
@@ -649,9 +650,10 @@ extern "C" int tt_sample_field(const float *field4, const tt_grid *grid, const d
 extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
                             int64_t ld_s0, int64_t n, const int32_t *order, double *rf, int64_t ld_rf,
                             double *phase, double *sf, int64_t ld_sf, int32_t *status, void *stream) {
-    TT_REQUIRE(field4 && grid && opts && s0 && rf, "tt_trace_rk4: null pointer");
+    TT_REQUIRE(grid && opts, "tt_trace_rk4: null grid or options");
     TT_REQUIRE(n >= 0, "tt_trace_rk4: n < 0");
-    if (n == 0) return TT_OK;
+    if (n == 0) return TT_OK;  // an empty ray set: nothing is read or written
+    TT_REQUIRE(field4 && s0 && rf, "tt_trace_rk4: null pointer");
     TT_REQUIRE(ld_s0 >= n && ld_rf >= n && (!sf || ld_sf >= n), "tt_trace_rk4: leading dimension < n");
     const double mm = 1.0 / opts->h;
     const int m = (int)llround(mm);

@@ -243,6 +243,8 @@ def path_estimate(field4, grid, s0, n=None, k=16, blocks=None, order=None, base=
         base = torch.empty((5, n), dtype=torch.float64, device=dev)
     if feat is None:
         feat = torch.empty((NFEAT, n), dtype=torch.float32, device=dev).t()
+    if n == 0:  # an empty ray set
+        return base, feat
     if tuple(base.shape) != (5, n) or not base.is_contiguous():
         raise ValueError("path_estimate: base must be a contiguous [5, n] tensor")
     if tuple(feat.shape) != (n, NFEAT) or feat.stride(0) != 1 or feat.stride(1) < max(n, 1):
@@ -420,6 +422,9 @@ class Surrogate:
         and fallback ms (and per-stage ms when chunks == 1)."""
         n = s0.shape[1]
         dev = self.device
+        if n == 0:  # an empty ray set
+            return (torch.empty((4, 0), dtype=torch.float64, device=dev), torch.empty(0, dtype=torch.float64, device=dev),
+                    torch.empty(0, dtype=torch.int32, device=dev))
         main = torch.cuda.current_stream()
         t0 = torch.cuda.Event(enable_timing=True)
         t0.record(main)
