@@ -47,7 +47,10 @@ def test_field_dropin_matches_reference_golden(lib, golden, N):
                                    (128, 96, 80),
                                    # the fixed-length passes along each axis
                                    (10, 6, 256), (12, 256, 8), (512, 6, 10), (6, 4, 1024),
-                                   (6, 8, 192), (384, 4, 6), (10, 128, 6)])
+                                   (6, 8, 192), (384, 4, 6), (10, 128, 6),
+                                   # reference sizes 2N + 1 with large prime factors (1025 = 5^2 41,
+                                   # 2049 = 3 683): direct-DFT passes
+                                   (1025, 3, 4), (3, 4, 2049)])
 def test_field_from_noise_any_shape(lib, shape):
     from turbulence_tracing_amd import turboGen
     rs = np.random.RandomState(sum(shape))
