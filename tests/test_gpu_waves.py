@@ -81,19 +81,33 @@ def test_large_volume_full_size_subset_matches_oracle(lib):
     RMS, waves of 1e7 rays from the 8 mm beam, 4096^2 detectors), two waves:
     256 consecutive rays of the last wave's entry-column order (one compact
     patch of columns) match the FP64 integrator on the node field cropped
-    around their paths, every ray of the wave leaves finite, and the
-    shadowgraph holds nearly every ray of both waves."""
+    around their paths, every ray of the wave leaves finite, the shadowgraph
+    holds nearly every ray of both waves, and both count histograms equal
+    the FP64 optics on all 2e7 exit rays exactly."""
     from oracle import tracer as OT
     from turbulence_tracing_amd.waves import WaveRun, large_volume_config
     cfg = large_volume_config()
     assert (cfg.n, cfg.pix) == (1024, 4096)
+    from oracle import rtm as OR
     run = WaveRun(cfg, 20_000_000, 10_000_000)
     run.field(7)
-    run.run(100)
+    waves_rf = []
+    run.run(100, on_wave=lambda w, n: waves_rf.append(run.p.rf[:, :n].cpu().numpy()))
     torch.cuda.synchronize()
     p = run.p
     assert bool(torch.isfinite(p.rf).all()) and bool(torch.isfinite(p.phase).all())
     assert int(p.H_shadow.sum()) > 0.99 * 20_000_000
+    # the 4096^2 count histograms over both waves equal the FP64 optics and
+    # np.histogram2d on the same 2e7 exit rays exactly
+    rf_all = np.concatenate(waves_rf, axis=1)
+    del waves_rf
+    Ws = OR.histogram(OR.shadowgraphy(rf_all, L=cfg.L_optics, R=cfg.R_optics), cfg.det_size, cfg.det_size, 1,
+                      cfg.pix, cfg.pix)[0]
+    np.testing.assert_array_equal(p.H_shadow.cpu().numpy(), Ws)
+    Wk = OR.histogram(OR.schlieren_knife(rf_all, 0.0, "x", 1, L=cfg.L_optics, R=cfg.R_optics), cfg.det_size,
+                      cfg.det_size, 1, cfg.pix, cfg.pix)[0]
+    np.testing.assert_array_equal(p.H_knife.cpu().numpy(), Wk)
+    del rf_all, Ws, Wk
     mid = run.wave // 2
     s0 = p.s0[:, mid:mid + 256].cpu().numpy()
     rf = p.rf[:, mid:mid + 256].cpu().numpy()
