@@ -106,13 +106,7 @@ def main():
     achieved = BYTES_PER_RAY_STEP * ray_steps / (trace_ms / 1e3) / 1e9
 
     if args.calib_copy:
-        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
-        log("calibration copy: 1 GiB read + 1 GiB written")
+        calibration_copy()
 
     # stage breakdown (separate, un-timed-region pass with events per stage)
     stages = {}
@@ -177,6 +171,19 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def calibration_copy():
+    """One device-to-device copy of 1 GiB, the reference dispatch that
+    FETCH_SIZE / WRITE_SIZE are scaled by."""
+    import torch
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    log("calibration copy: 1 GiB read + 1 GiB written")
+
+
 def main_large(args):
     """1024^3 FP32 field (L = 20 mm, 15% RMS, ne 5e24), 1e8 rays in an 8 mm
     disc streamed in 1e7-ray waves, 4096^2 histograms.  A step is the whole
@@ -211,6 +218,8 @@ def main_large(args):
     b.record(stream)
     torch.cuda.synchronize()
     trace_ms = a.elapsed_time(b)
+    if args.calib_copy:
+        calibration_copy()
     wave_steps = run.wave * cfg.steps_per_ray
     achieved = BYTES_PER_RAY_STEP * wave_steps / (trace_ms / 1e3) / 1e9
     out = {
@@ -226,8 +235,10 @@ def main_large(args):
                    "grid": cfg.n, "rays": run.total, "wave": run.wave, "steps_per_ray": cfg.steps_per_ray},
         "rays_per_s": run.total / (ms_per_step / 1e3),
         "roofline": {"kernel": "trace_rk4_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None, "trace_ms": trace_ms,
-                     "bytes_per_ray_step": BYTES_PER_RAY_STEP},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(run.p.cfg),
+                     "traffic_source": "profiles/trace_rk4_1024_1e7_hbm_traffic.md (FETCH_SIZE x 2 + WRITE_SIZE x 1, "
+                                       "calibrated on a 1 GiB copy)",
+                     "trace_ms": trace_ms, "bytes_per_ray_step": BYTES_PER_RAY_STEP},
         "cpu_baseline": None,
     }
     if not args.no_surrogate:
@@ -296,13 +307,17 @@ def surrogate_large(run, cfg, plain_ms):
     return res
 
 
+TRAFFIC_PROFILES = {(512, 0): "trace_rk4_512_1e7_hbm_traffic.md", (1024, 0): "trace_rk4_1024_1e7_hbm_traffic.md"}
+
+
 def profiled_traffic(cfg):
-    """HBM bytes per trace launch measured by rocprofv3 (FETCH_SIZE and
-    WRITE_SIZE, calibrated on a plain copy) for the 512^3 / 1e7-ray workload,
-    read from its committed summary; None for other workloads."""
-    if (cfg.n, cfg.n_rays, cfg.variant) != (512, 10_000_000, 0):
+    """HBM bytes per 1e7-ray trace launch measured by rocprofv3 (FETCH_SIZE
+    and WRITE_SIZE, calibrated on a plain copy) for the 512^3 and the 1024^3
+    workloads, read from their committed summaries; None for others."""
+    name = TRAFFIC_PROFILES.get((cfg.n, cfg.variant))
+    if name is None or cfg.n_rays != 10_000_000:
         return None
-    path = os.path.join(ROOT, "profiles", "trace_rk4_512_1e7_hbm_traffic.md")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("trace_rk4_traffic_bytes:"):
