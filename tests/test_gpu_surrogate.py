@@ -179,8 +179,8 @@ def test_path_features_match_oracle(lib, k, inside):
 @pytest.mark.parametrize("inside", [False, True])
 def test_path_k1_node_field_matches_block_fields(lib, inside):
     """At k = 1 the estimate reads the node field (no block fields) and
-    interpolates the sums and differences of the two node planes; it must
-    agree with the k = 1 block-field path to float rounding, on a wide beam
+    evaluates each node plane's bilinear form; it must agree with the k = 1
+    block-field path to float rounding, on a wide beam
     whose rays change column many times (a ray within rounding of a column
     boundary may count one column change more or less)."""
     import torch
