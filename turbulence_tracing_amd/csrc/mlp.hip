@@ -356,12 +356,16 @@ __device__ __forceinline__ void pipe_layer(const Pipe<G> &P, const float *bias, 
 #pragma unroll
                 for (int t = 0; t < T; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
             }
+            // raised wave priority while this wave issues its MFMA chain, so the
+            // SIMD's other waves' epilogues fill the gaps rather than delay it
+            __builtin_amdgcn_s_setprio(1);
 #pragma unroll
             for (int cc = 0; cc < KC; ++cc) {
                 const bf16x8 a = *reinterpret_cast<const bf16x8 *>(lb + ((tl * KC + cc) * 64 + lane) * 8);
 #pragma unroll
                 for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[cc][t], acc[t], 0, 0, 0);
             }
+            __builtin_amdgcn_s_setprio(0);
             if (GELU) {
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
