@@ -366,9 +366,14 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
         } else {  // whole cells from the cache
             for (;;) {
                 int ix, iy;
+                // raised wave priority while the cell's loads are issued, so
+                // they leave early and their latency hides behind other waves'
+                // arithmetic (raising it for the gathered redo instead was slower)
+                __builtin_amdgcn_s_setprio(1);
                 locate(ix, iy);
                 ZP z0, D;
                 load_planes(a, ix, iy, k >> 1, z0, D);
+                __builtin_amdgcn_s_setprio(0);
                 float mn = inside ? 0.f : -1.f, mx = 0.f;
                 const ZP zq = zstep(z0, 0.25f, D);
                 const ZP zh = zstep(z0, 0.5f, D);
