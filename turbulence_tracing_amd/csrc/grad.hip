@@ -61,11 +61,13 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs<T> a) {
     T zm = z0 > 0 ? load_s(a, beta, ix, iy, z0 - 1) : (T)0;
     T s0 = load_s(a, beta, ix, iy, z0);
     for (int iz = z0; iz < z1; ++iz) {
+        __builtin_amdgcn_s_setprio(1);  // the plane's loads leave before other waves' arithmetic
         const T zp = iz < a.nz - 1 ? load_s(a, beta, ix, iy, iz + 1) : s0;
         const T xm = ix > 0 ? load_s(a, beta, ix - 1, iy, iz) : s0;
         const T xp = ix < a.nx - 1 ? load_s(a, beta, ix + 1, iy, iz) : s0;
         const T ym = iy > 0 ? load_s(a, beta, ix, iy - 1, iz) : s0;
         const T yp = iy < a.ny - 1 ? load_s(a, beta, ix, iy + 1, iz) : s0;
+        __builtin_amdgcn_s_setprio(0);
         float4 o;
         o.x = (float)grad1(xm, s0, xp, ix, a.nx, a.cx);
         o.y = (float)grad1(ym, s0, yp, iy, a.ny, a.cy);
