@@ -142,7 +142,7 @@ int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *
  * traced in ONE persistent launch.  No reference counterpart as a call: it
  * replaces running ElectronCube.solve once per realisation, as the reference's
  * drivers do in a loop or across MPI ranks (particle_tracking/
- * multiprocessing_example.py:41-62, mpi_example.py:82-149).  Each item is one
+ * example_multiprocess.py:41-62, example_MPI.py:82-149).  Each item is one
  * tt_trace_rk4 call's arguments (no sf / status).  tt_batch_prepare uploads the
  * items and a chunk table (chunk rays per entry, items with the most z-steps
  * first) into `workspace` with synchronous copies and fills `info`;

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY (see ``oracle/__init__``).
 
 Reference: particle_tracking/particle_tracker.py (PT below).
   * ``Cube``                 PT:121-145, 212-218 (construction, external_ne)
+  * ``Cube.test_*``          PT:147-210 (density builders)
   * ``Cube.calc_dndr``       PT:220-241 (omega, n_c, clip, np.gradient, RGIs)
   * ``Cube.dndr``            PT:243-256
   * ``Cube.init_beam``       PT:258-310 (draw order t, u, u, phi, chi)
@@ -16,6 +17,11 @@ Integrators that are not in the reference (they define the accuracy bar):
   * ``zmarch_rk4``           the fixed-step RK4 the GPU tracer runs, in FP64:
                              z is the independent variable, steps of h cells
                              land exactly on grid planes and on the exit plane.
+  * ``FieldCube``            a Cube over given node fields (a window of a
+                             device node field) for ``solve_tight``.
+Pins: tests/golden/reference_vectors.npz and builder_vectors.npz (written
+from the reference's own functions), the null / slab / uniform-slab closed
+forms, and solve_tight against the z-march (tests/test_oracle_golden.py).
 """
 import numpy as np
 import scipy.constants as sc
@@ -48,6 +54,36 @@ class Cube:
 
     def external_ne(self, ne):
         self.ne = ne
+
+    # density builders, PT:147-210 (meshgrid 'ij', extents from the axes)
+    def _mesh(self):
+        return np.meshgrid(self.x, self.y, self.z, indexing="ij")
+
+    def test_null(self):
+        XX, _, _ = self._mesh()
+        self.ne = np.zeros_like(XX)
+
+    def test_slab(self, s=1, n_e0=2e23):
+        XX, _, _ = self._mesh()
+        self.ne = n_e0 * (1.0 + s * XX / self.extent_x)
+
+    def test_linear_cos(self, s1=0.1, s2=0.1, n_e0=2e23, Ly=1):
+        XX, YY, _ = self._mesh()
+        self.ne = n_e0 * (1.0 + s1 * XX / self.extent_x) * (1 + s2 * np.cos(2 * np.pi * YY / Ly))
+
+    def test_exponential_cos(self, n_e0=1e24, Ly=1e-3, s=2e-3):
+        XX, YY, _ = self._mesh()
+        self.ne = n_e0 * 10 ** (XX / s) * (1 + np.cos(2 * np.pi * YY / Ly))
+
+    def test_lens(self, n_e0=1e24, LR=1e-3):
+        XX, YY, _ = self._mesh()
+        RR = np.sqrt(XX ** 2 + YY ** 2)
+        self.ne = n_e0 * np.exp(-RR ** 2 / LR ** 2)
+
+    def test_liner(self, n_e0=1e24, LR=1e-3):
+        XX, _, ZZ = self._mesh()
+        RR = np.sqrt(XX ** 2 + ZZ ** 2)
+        self.ne = n_e0 * np.exp(-RR ** 2 / LR ** 2)
 
     def calc_dndr(self, lwl=1053e-9, ne_max=1):
         self.lwl = lwl
@@ -330,12 +366,42 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
     return rf, phase, sf, status
 
 
+class FieldCube(Cube):
+    """A Cube whose node fields are given directly (gamma_x, gamma_y, gamma_z
+    in 1/m, i.e. dndr / c^2, and eta = n - 1 on the nodes of the x, y, z
+    axes), e.g. a window of a device node field; calc_dndr is not run.  Its
+    dndr and interpolators are the RegularGridInterpolators of PT:239-241 over
+    c^2 gamma."""
+
+    def __init__(self, x, y, z, gx, gy, gz, eta, lwl, probing_direction="z"):
+        super().__init__(np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64),
+                         np.asarray(z, dtype=np.float64), probing_direction)
+        self.lwl = lwl
+        self.dndx, self.dndy, self.dndz = gx * C ** 2, gy * C ** 2, gz * C ** 2
+        self._eta = eta
+        g = (self.x, self.y, self.z)
+        self.dndx_interp = RegularGridInterpolator(g, self.dndx, bounds_error=False, fill_value=0.0)
+        self.dndy_interp = RegularGridInterpolator(g, self.dndy, bounds_error=False, fill_value=0.0)
+        self.dndz_interp = RegularGridInterpolator(g, self.dndz, bounds_error=False, fill_value=0.0)
+        self.extent = self._extent_for_beam()
+
+    def node_fields(self):
+        return self.dndx / C ** 2, self.dndy / C ** 2, self.dndz / C ** 2, self._eta
+
+
 def solve_tight(cube, s0, k0=None, rtol=1e-11, atol_pos=1e-14, atol_vel=1e-6, atol_phase=1e-10,
-                max_step=np.inf):
-    """Per-ray RK45 at tight tolerance on the reference's own RHS (dsdt via
-    RegularGridInterpolator), with the phase as a 7th component
-    d phi/dt = k0 * eta(x) * |v|.  Integrates to t = sqrt(8) extent / c as
-    PT:317 does, then ray_at_exit.  Returns rf (4 x N), phase (N), nfev."""
+                max_step=np.inf, bundle=False):
+    """RK45 at tight tolerance on the reference's own RHS (dsdt: the
+    gradient RegularGridInterpolators of calc_dndr, PT:239-241, 398-419),
+    with the phase as a 7th component d phi/dt = k0 * eta(x) * |v|.
+    Integrates to t = sqrt(8) extent / c as PT:317 does, then ray_at_exit.
+
+    bundle=False: one solve_ivp per ray.  bundle=True: one solve_ivp over
+    all rays together, as the reference's solve does (PT:323), but at the
+    tight tolerance; the error norm then holds every ray to it, and since
+    all rays cross the z-planes (where the trilinear field has its kinks) at
+    nearly the same times, the bundle costs about as much as a few single
+    rays.  Returns rf (4 x N), phase (N), sf (6 x N), nfev."""
     if k0 is None:
         k0 = wavenumber(cube.lwl)
     gx, gy, gz, eta = cube.node_fields()
@@ -345,21 +411,26 @@ def solve_tight(cube, s0, k0=None, rtol=1e-11, atol_pos=1e-14, atol_vel=1e-6, at
     sf = np.zeros((6, N))
     phase = np.zeros(N)
     nfev = 0
-    atol = np.array([atol_pos] * 3 + [atol_vel] * 3 + [atol_phase])
 
     def f(t, y):
-        out = np.empty(7)
-        p = y[:3].reshape(3, 1)
-        out[:3] = y[3:6]
-        out[3:6] = cube.dndr(p)[:, 0]
-        out[6] = k0 * eta_i(p.T)[0] * np.sqrt(y[3] ** 2 + y[4] ** 2 + y[5] ** 2)
+        n = y.size // 7
+        out = np.empty(7 * n)
+        p = y[:3 * n].reshape(3, n)
+        v = y[3 * n:6 * n].reshape(3, n)
+        out[:3 * n] = y[3 * n:6 * n]
+        out[3 * n:6 * n] = cube.dndr(p).ravel()
+        out[6 * n:] = k0 * eta_i(p.T) * np.sqrt(v[0] ** 2 + v[1] ** 2 + v[2] ** 2)
         return out
 
-    for i in range(N):
-        y0 = np.concatenate([s0[:, i], [0.0]])
+    groups = [np.arange(N)] if bundle else [np.array([i]) for i in range(N)]
+    for idx in groups:
+        n = idx.size
+        atol = np.repeat(np.array([atol_pos] * 3 + [atol_vel] * 3 + [atol_phase]), n)
+        y0 = np.concatenate([s0[:, idx].ravel(), np.zeros(n)])
         sol = solve_ivp(f, [0, T], y0, method="RK45", rtol=rtol, atol=atol, max_step=max_step)
-        sf[:, i] = sol.y[:6, -1]
-        phase[i] = sol.y[6, -1]
+        yT = sol.y[:, -1]
+        sf[:, idx] = yT[:6 * n].reshape(6, n)
+        phase[idx] = yT[6 * n:]
         nfev += sol.nfev
     rf = ray_at_exit(sf, cube.extent, cube.probing_direction)
     return rf, phase, sf, nfev

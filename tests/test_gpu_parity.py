@@ -109,7 +109,9 @@ def test_grad_stencil_matches_calc_dndr(lib, golden):
         got = getattr(cube, k).cpu().numpy()
         want = golden["cube_" + k]
         assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
-    # float32 canonical-layout input with the device-side RMS normalisation
+    # float32 canonical-layout input through the drop-in's calc_dndr (no
+    # normalisation: the density is passed as it is); the Pipeline's fused
+    # unit-RMS normalisation and density map are pinned in test_gpu_pins.py
     n = 24
     xx, ne = small_cube(n)
     ref = OT.Cube(xx, xx.copy(), xx.copy())
@@ -562,7 +564,7 @@ def test_probing_x_rays_start_past_the_cube(lib):
 
 
 def test_detectors_pickle_like_the_reference(lib, tmp_path):
-    """Diagnostics pickle (mpi_example.py:154-166) to numpy-backed objects
+    """Diagnostics pickle (example_MPI.py:154-166) to numpy-backed objects
     that load without touching the GPU and keep H, edges and rays."""
     import pickle
     from turbulence_tracing_amd import ray_transfer_matrix as rtm

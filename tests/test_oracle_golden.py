@@ -228,6 +228,75 @@ def test_zmarch_matches_tight_solver_turbulent():
     np.testing.assert_allclose(sf[:2], sft[:2], atol=1e-12)
 
 
+def tilted_beam(n_rays, lo, hi, theta, seed):
+    """Rays from z = -extent with launch angles theta (rad, x and y) plus a
+    small jitter, positions uniform in [lo, hi]^2."""
+    rs = np.random.RandomState(seed)
+    s0 = np.zeros((6, n_rays))
+    s0[0] = rs.uniform(lo, hi, n_rays)
+    s0[1] = rs.uniform(lo, hi, n_rays)
+    s0[2] = -5e-3
+    tx = theta[0] + rs.normal(0, 0.01, n_rays)
+    ty = theta[1] + rs.normal(0, 0.01, n_rays)
+    vz = 1.0 / np.sqrt(1 + np.tan(tx) ** 2 + np.tan(ty) ** 2)
+    s0[3], s0[4], s0[5] = T.C * vz * np.tan(tx), T.C * vz * np.tan(ty), T.C * vz
+    return s0
+
+
+def turbulent_cube(n, eps, seed=2):
+    x = np.linspace(-5e-3, 5e-3, n)
+    d = F.powerlaw_field((n, n, n), -11 / 3, seed=seed)
+    d /= np.sqrt((d ** 2).mean())
+    cube = T.Cube(x, x.copy(), x.copy())
+    cube.external_ne(1e24 * (1 + eps * d))
+    cube.calc_dndr(lwl=1064e-9)
+    cube.extent = cube.extent_z
+    return x, cube
+
+
+def test_tight_bundle_equals_per_ray():
+    """One tight solve_ivp over a bundle of rays gives every ray the
+    per-ray tight answer."""
+    x, cube = turbulent_cube(13, 0.1)
+    s0 = tilted_beam(5, -3e-3, 3e-3, (0.05, -0.04), seed=3)
+    r1, p1, s1, _ = T.solve_tight(cube, s0)
+    r2, p2, s2, _ = T.solve_tight(cube, s0, bundle=True)
+    dx = x[1] - x[0]
+    assert np.abs(r1[[0, 2]] - r2[[0, 2]]).max() < 1e-9 * dx
+    # both are tight to ~1e-10 rad in angle (rtol 1e-11 on |v| = c per step,
+    # summed over the steps) and ~1e-8 relative in phase, far inside the
+    # 1e-3 of the deflection and 1e-5 of the phase the z-march is held to
+    assert np.abs(r1[[1, 3]] - r2[[1, 3]]).max() < 2e-9
+    assert np.abs(p1 - p2).max() < 1e-7 * np.abs(p1).max()
+
+
+def test_zmarch_matches_tight_solver_across_cell_faces():
+    """The fixed-step z-march (h = 1/2 cell, what the GPU runs) against tight
+    solve_ivp on a 33^3 turbulent cube (20% RMS) with rays tilted ~0.15 rad
+    so that every ray crosses several x and y cell faces, where the
+    trilinear field has kinks and RK4 loses its order.  The bar of the path:
+    exit positions within 1e-3 cells, exit angles within 1e-3 of the RMS
+    deflection (exit minus launch angle) and the phase within 1e-5 of its
+    magnitude."""
+    n = 33
+    x, cube = turbulent_cube(n, 0.2)
+    dx = x[1] - x[0]
+    s0 = tilted_beam(8, -3e-3, 3e-3, (0.15, -0.12), seed=4)
+    rf, ph, sf, st = T.zmarch_rk4(T.NodeGrid(cube), s0)
+    rt, pt_, sft, _ = T.solve_tight(cube, s0, bundle=True)
+    assert (st == 0).all()
+    # every ray crossed at least 3 faces in x and 3 in y
+    cx = np.abs(np.floor((rt[0] - x[0]) / dx) - np.floor((s0[0] - x[0]) / dx))
+    cy = np.abs(np.floor((rt[2] - x[0]) / dx) - np.floor((s0[1] - x[0]) / dx))
+    assert cx.min() >= 3 and cy.min() >= 3
+    launch = np.arctan(s0[[3, 4]] / s0[5])
+    defl = np.sqrt(np.mean((rt[[1, 3]] - launch) ** 2))
+    assert defl > 1e-4
+    assert np.abs(rf[[0, 2]] - rt[[0, 2]]).max() <= 1e-3 * dx
+    assert np.abs(rf[[1, 3]] - rt[[1, 3]]).max() <= 1e-3 * defl
+    assert np.abs(ph - pt_).max() <= 1e-5 * np.abs(pt_).max()
+
+
 def test_zmarch_partial_first_step_and_outside_start():
     n = 9
     x = np.linspace(-5e-3, 5e-3, n)
@@ -299,3 +368,58 @@ def test_knife_edge_fourier_plane_closed_form():
     rf = R.schlieren_knife(r0, offset=0.3, axis="x", direction=1, L=400, R=1e9, focal_plane=0.7)
     passed = ~np.isnan(rf[0])
     np.testing.assert_array_equal(passed, ~(400 * r0[1] > 0.3))
+
+
+# ------------------------------------------- builders, probing axes, rect
+@pytest.fixture(scope="module")
+def builders():
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "builder_vectors.npz"))
+
+
+@pytest.mark.parametrize("key,fn,kw", [
+    ("null", "test_null", {}),
+    ("slab", "test_slab", {"s": 3, "n_e0": 5e23}),
+    ("linear_cos", "test_linear_cos", {"s1": 0.3, "s2": 0.2, "n_e0": 4e23, "Ly": 3e-3}),
+    ("exponential_cos", "test_exponential_cos", {"n_e0": 2e24, "Ly": 2e-3, "s": 4e-3}),
+    ("lens", "test_lens", {"n_e0": 3e24, "LR": 2e-3}),
+    ("liner", "test_liner", {"n_e0": 3e24, "LR": 2e-3})])
+def test_density_builders_golden(builders, key, fn, kw):
+    cube = T.Cube(builders["b_x"], builders["b_y"], builders["b_z"])
+    getattr(cube, fn)(**kw)
+    np.testing.assert_array_equal(cube.ne, builders[f"b_{key}"])
+
+
+def test_rect_aperture_golden(builders):
+    got = R.rect_aperture(builders["rect_r"].copy(), 2.0, 1.5)
+    np.testing.assert_array_equal(got, builders["rect_out"])
+    # the reference rejects a ray only when it is outside in x AND in y
+    r = builders["rect_r"]
+    np.testing.assert_array_equal(np.isnan(got[0]), (np.abs(r[0]) > 2.0) & (np.abs(r[2]) > 1.5))
+
+
+@pytest.mark.parametrize("d", ["y", "x"])
+def test_probing_axis_beam_and_as_shipped_solve_golden(builders, d):
+    cube = T.Cube(builders["p_x"], builders["p_y"], builders["p_z"], probing_direction=d)
+    cube.external_ne(builders["p_ne"].copy())
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(32, 1.5e-3, 5e-3, rng=np.random.RandomState(int(builders[f"p_{d}_seed"])))
+    np.testing.assert_array_equal(s0, builders[f"p_{d}_s0"])
+    rf = cube.solve()
+    np.testing.assert_allclose(cube.sf, builders[f"p_{d}_sf"], rtol=1e-12, atol=1e-18)
+    np.testing.assert_allclose(rf, builders[f"p_{d}_rf"], rtol=1e-10, atol=1e-15)
+
+
+def test_probing_y_tight_solve_near_as_shipped(builders):
+    """The tight per-ray solve with probing along y sits within the
+    as-shipped RK45's own tolerance band of the reference's trace."""
+    cube = T.Cube(builders["p_x"], builders["p_y"], builders["p_z"], probing_direction="y")
+    cube.external_ne(builders["p_ne"].copy())
+    cube.calc_dndr(lwl=1064e-9)
+    cube.extent = cube.extent_y
+    s0 = builders["p_y_s0"]
+    rt, _, _, _ = T.solve_tight(cube, s0[:, :8], bundle=True)
+    ref = builders["p_y_rf"][:, :8]
+    dx = builders["p_x"][1] - builders["p_x"][0]
+    assert np.abs(rt[[0, 2]] - ref[[0, 2]]).max() < 1e-2 * dx
+    assert np.abs(rt[[1, 3]] - ref[[1, 3]]).max() < 1e-2 * np.abs(ref[[1, 3]]).max()

@@ -3,7 +3,7 @@ one persistent launch, the whole ensemble step captured in one HIP graph.
 
 The reference runs realisations one after another (its drivers loop over
 seeds or spread them over processes, particle_tracking/
-multiprocessing_example.py:41-62, mpi_example.py:82-149), each through
+example_multiprocess.py:41-62, example_MPI.py:82-149), each through
 gaussian3D_FFT, calc_dndr, init_beam, solve and the detectors.  Here every
 member keeps its own resident buffers (a Pipeline), the per-member stages
 (field, gradient, beam, ray order, detectors) are stream-ordered launches, and

@@ -19,8 +19,10 @@ What changes underneath:
   * init_beam draws from numpy's global RNG exactly as the reference does
     (so seeded runs give the same rays) unless ``seed=`` is given, in which
     case the GPU Philox sampler draws the same distribution on the device.
-Only probing_direction 'z' is traced (the tracer marches along z);
-'x'/'y' raise NotImplementedError in solve.
+  * probing_direction 'x' and 'y' are traced in an axis-permuted frame whose
+    march axis is the probing axis (the node field of that frame is built by
+    the stencil from a strided view of the density), so the exit rays are
+    the reference's ray_at_exit rows for that direction (PT:353-368).
 """
 import time
 from datetime import datetime
@@ -98,12 +100,14 @@ class ElectronCube:
     def test_lens(self, n_e0=1e24, LR=1e-3):
         """PT:190-199."""
         XX, YY, _ = self._mesh()
-        self.ne = n_e0 * torch.exp(-(XX ** 2 + YY ** 2) / LR ** 2)
+        RR = torch.sqrt(XX ** 2 + YY ** 2)
+        self.ne = n_e0 * torch.exp(-RR ** 2 / LR ** 2)
 
     def test_liner(self, n_e0=1e24, LR=1e-3):
         """PT:201-210."""
         XX, _, ZZ = self._mesh()
-        self.ne = n_e0 * torch.exp(-(XX ** 2 + ZZ ** 2) / LR ** 2)
+        RR = torch.sqrt(XX ** 2 + ZZ ** 2)
+        self.ne = n_e0 * torch.exp(-RR ** 2 / LR ** 2)
 
     def external_ne(self, ne):
         """Load an externally generated (Mx, My, Mz) density grid, m^-3 (PT:212-218)."""
@@ -147,6 +151,22 @@ class ElectronCube:
     @property
     def dndz(self):
         return self._channel_ij(2).double() * (c ** 2 / self.grid.dz)
+
+    @property
+    def dndx_interp(self):
+        """Callable like the RegularGridInterpolator of PT:239: values of
+        dndx at points xi of shape (..., 3), zero outside the grid."""
+        return _ChannelInterp(self, 0)
+
+    @property
+    def dndy_interp(self):
+        """As dndx_interp, for dndy (PT:240)."""
+        return _ChannelInterp(self, 1)
+
+    @property
+    def dndz_interp(self):
+        """As dndx_interp, for dndz (PT:241)."""
+        return _ChannelInterp(self, 2)
 
     @property
     def ne_nc(self):
@@ -312,6 +332,27 @@ class ElectronCube:
         with open(fn, "wb") as f:
             np.save(f, self.rf.cpu().numpy())
         return fn
+
+
+class _ChannelInterp:
+    """One gradient channel of the node field as a callable with the
+    RegularGridInterpolator call convention of PT:239-241 and PT:253-255:
+    interp(xi) with xi of shape (..., 3) (metres) returns an array of shape
+    (...) of dndx / dndy / dndz (m/s^2), linear inside the grid, zero
+    strictly outside it and NaN at NaN points (tt_sample_field)."""
+
+    def __init__(self, cube, channel):
+        self.cube = cube
+        self.channel = channel
+
+    def __call__(self, xi):
+        xi = _dev(xi, device=self.cube.device)
+        if xi.shape[-1] != 3:
+            raise ValueError(f"points must have shape (..., 3), got {tuple(xi.shape)}")
+        lead = xi.shape[:-1]
+        pts = xi.reshape(-1, 3).T.contiguous()
+        g = self.cube.sample_field(pts)[self.channel]
+        return (g * (c ** 2 / self.cube.grid.dz)).reshape(lead)
 
 
 def dsdt(t, s, ElectronCube):

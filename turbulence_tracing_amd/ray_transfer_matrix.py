@@ -191,7 +191,7 @@ class Rays:
         self.r0 = None
         self.rf = None
 
-    # The reference's drivers pickle whole diagnostics (mpi_example.py:154-166).
+    # The reference's drivers pickle whole diagnostics (example_MPI.py:154-166).
     # Device tensors are stored as numpy arrays, so a pickle loads on a host
     # without a GPU and reads like the reference's (H, rays as numpy).
     def __getstate__(self):

@@ -8,7 +8,7 @@ of one large draw), orders them by entry column, traces them with the plain
 RK4 tracer or the surrogate + fallback form, and adds them into the detector
 histograms.  The reference does the same work in one solve over all rays
 (particle_tracking/particle_tracker.py:312-331) or split across processes
-whose histograms are summed (particle_tracking/mpi_example.py:136-149).
+whose histograms are summed (particle_tracking/example_MPI.py:136-149).
 """
 from .pipeline import Config, Pipeline
 
