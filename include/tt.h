@@ -198,6 +198,16 @@ int tt_beam_sample(int64_t n, double beam_size, double divergence, double extent
 int tt_ray_sort_scratch_bytes(const tt_grid *grid, int32_t brick_shift, size_t *bytes);
 int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
                 void *scratch, int32_t *order, void *stream);
+/* The same ordering with no scratch argument and no allocation: no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops).  The column
+ * counts live in a static device array of the library, 4 MiB per device
+ * (2^20 entry-column bins: grids up to 1024 columns per axis after
+ * brick_shift); larger grids are rejected with TT_ERR_ARG.  Every call on one
+ * device shares that array, so calls that overlap on one device (on
+ * different streams) race; calls queued on one stream do not. */
+int tt_ray_sort_noalloc(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
+                        int32_t *order, void *stream);
 /* The same ordering, also moving the rays: out (6 x ldo float64) = s0's
  * columns in `order`.  staging (device, 8 n float64, 16-byte aligned) holds
  * one 64-byte record per ray in between, so the scattered writes are whole

@@ -96,6 +96,7 @@ _SIGNATURES = {
     "tt_beam_sample": [I64, D, D, D, I32, U64, U64, P, I64, P],
     "tt_ray_sort_scratch_bytes": [C.POINTER(Grid), I32, SIZE_P],
     "tt_ray_sort": [P, I64, I64, C.POINTER(Grid), I32, P, P, P],
+    "tt_ray_sort_noalloc": [P, I64, I64, C.POINTER(Grid), I32, P, P],
     "tt_ray_sort_rays": [P, I64, I64, C.POINTER(Grid), I32, P, P, P, P, I64, P],
     "tt_optics_apply": [C.POINTER(Train), P, I64, I64, P, I64, P],
     "tt_detect": [C.POINTER(Detector), I32, P, I64, P, I64, P],

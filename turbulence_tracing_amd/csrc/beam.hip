@@ -1,6 +1,8 @@
 // Beam sampler (the distribution of ElectronCube.init_beam,
 // particle_tracking/particle_tracker.py:258-310) and the ray ordering used by
 // the tracer (a counting sort by the Morton code of the entry column).
+#include <mutex>
+
 #include "tt_internal.h"
 
 namespace {
@@ -246,6 +248,59 @@ int sort_impl(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int3
 extern "C" int tt_ray_sort(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
                            void *scratch, int32_t *order, void *stream) {
     return sort_impl(s0, ld, n, grid, brick_shift, scratch, order, nullptr, nullptr, 0, stream);
+}
+
+// ------------------------------------------------- allocation-free ordering
+// tt_ray_sort_noalloc keeps its column counts and scan partials in this
+// file-scope array instead of a caller-provided scratch buffer: room for
+// 2^20 entry columns (Morton keys of up to 1024 x 1024 columns, i.e. grids up
+// to 1024 nodes per axis at brick_shift 0).  Each device holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns on that device.
+constexpr int64_t kStaticSortBins = (int64_t)1 << 20;
+constexpr int64_t kStaticSortWords = kStaticSortBins + 2 * (kStaticSortBins / kScanChunk) + 64;
+__device__ __align__(256) uint32_t scratch[kStaticSortWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// the address of `scratch` on the current device, looked up once per device
+int static_sort_scratch(void **out) {
+    int dev = -1;
+    TT_HIP(hipGetDevice(&dev), "tt_ray_sort_noalloc: hipGetDevice");
+    TT_REQUIRE(dev >= 0 && dev < kMaxDevices, "tt_ray_sort_noalloc: device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        TT_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "tt_ray_sort_noalloc: hipGetSymbolAddress");
+        TT_REQUIRE(ptr, "tt_ray_sort_noalloc: null symbol address");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return TT_OK;
+}
+}  // namespace
+
+extern "C" int tt_ray_sort_noalloc(const double *s0, int64_t ld, int64_t n, const tt_grid *grid,
+                                   int32_t brick_shift, int32_t *order, void *stream) {
+    SortGeom g;
+    int64_t nbins;
+    int rc = sort_geom(grid, brick_shift, g, nbins);
+    if (rc) return rc;
+    TT_REQUIRE(nbins <= kStaticSortBins,
+               "tt_ray_sort_noalloc: %lld entry-column bins exceed the static scratch of %lld (grids up to "
+               "1024 columns per axis after brick_shift); use tt_ray_sort with a scratch buffer",
+               (long long)nbins, (long long)kStaticSortBins);
+    TT_REQUIRE(n >= 0 && ld >= n, "tt_ray_sort_noalloc: bad sizes");
+    if (n == 0) return TT_OK;  // an empty ray set: nothing is read or written
+    void *ptr = nullptr;
+    rc = static_sort_scratch(&ptr);
+    if (rc) return rc;
+    // sort_impl clears the counts with a kernel before counting, and the scan
+    // writes every partial before it is read
+    return sort_impl(s0, ld, n, grid, brick_shift, ptr, order, nullptr, nullptr, 0, stream);
 }
 
 extern "C" int tt_ray_sort_rays(const double *s0, int64_t ld, int64_t n, const tt_grid *grid, int32_t brick_shift,
