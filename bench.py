@@ -9,16 +9,28 @@ the shadowgraph / knife-edge schlieren / interferogram histograms (2048^2).
 Everything is resident in HBM; the timed region has no host transfers.
 
 value = ray-steps/s of the whole step (rays x steps per ray / step time).
-roofline = the trace kernel (the dominant kernel): 512 algorithmic bytes per
-ray-step (4 RK4 stages x 8 trilinear corners x 16 B float4) over its average
-duration from HIP events on the launch stream, against 8 TB/s.
-cpu_baseline = the reference's CPU path (as-shipped solve_ivp RK45 through
-RegularGridInterpolator, multiprocessing pool) timed on this host on a sample
-of the same rays through the same field.
-surrogate = the learned-correction form on the same realisation: z-block path
-estimate + MLP + RK4 fallback on flagged rays, its time against the plain
-trace (the speedup), its error against the plain trace of the same rays, and
-what the network gains over the path estimate alone (error RMS ratios).
+roofline = the trace kernel (the dominant kernel, ~93% of the step), which is
+bound by VALU instruction issue: its VALU wave-instructions per launch (the
+per-wave-step count measured by rocprofv3 SQ_INSTS_VALU on this binary, times
+the launch's wave-steps) over its average duration from HIP events on the
+launch stream, as the FP32 rate they carry at 2 FLOP per lane (64 lanes per
+wave-instruction), against MI355X's 157.3 TFLOP/s FP32 vector peak.  Its HBM
+side is reported beside it: the measured bytes per launch (FETCH_SIZE and
+WRITE_SIZE, calibrated on a plain copy) over the same duration, against
+8 TB/s, and the 512 algorithmic gather bytes per ray-step (4 RK4 stages x 8
+trilinear corners x 16 B), which registers and L1 serve.
+cpu_baseline = the reference's CPU path (solve_ivp RK45 through
+RegularGridInterpolator on a multiprocessing pool) timed on this host on
+samples of the same rays through the same field, as shipped (one RK45 per
+bundle, rtol 1e-3) and at the stated tolerance (one RK45 per ray at
+rtol 1e-6), each with its error against the GPU trace of the same rays.
+surrogate = the learned-correction form at its stated scale: trained on 2e7
+rays through 64 seeded 512^3 realisations, error heads calibrated on one more,
+evaluated on 1e8 rays over 8 held-out realisations; z-block path estimate +
+MLP + RK4 fallback on flagged rays, its time against the plain trace of the
+same rays (the speedup), its errors against that trace (max, p99.9, RMS), the
+fallback fraction, and what the network gains over the path estimate alone
+(error RMS ratios).
 """
 import argparse
 import json
@@ -30,6 +42,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak: 2 FLOP x 64 lanes per VALU wave-instruction
+BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 MFMA peak
 BYTES_PER_RAY_STEP = 512
 
 
@@ -40,15 +54,18 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--rays", type=float, default=1e7)
-    ap.add_argument("--cpu-bundle", type=int, default=500, help="rays per process per CPU round")
-    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU rounds")
+    ap.add_argument("--cpu-bundle", type=int, default=500, help="rays per process per as-shipped CPU round")
+    ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU rounds per mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="tracer kernel: 0 cell-cached, 1 per-stage gathers")
     ap.add_argument("--workload", choices=["single", "large"], default="single",
                     help="single: 512^3 / 1e7 rays (default); large: 1024^3 / 1e8 rays streamed in 1e7-ray waves")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate + fallback leg")
-    ap.add_argument("--sur-fields", type=int, default=48, help="training realisations for the surrogate")
-    ap.add_argument("--sur-block", type=int, default=16, help="z-block size (cells) of the path estimate")
+    ap.add_argument("--sur-fields", type=int, default=64, help="training realisations for the surrogate")
+    ap.add_argument("--sur-rays", type=int, default=312_500, help="training rays per realisation")
+    ap.add_argument("--sur-eval-fields", type=int, default=8, help="held-out evaluation realisations")
+    ap.add_argument("--sur-eval-rays", type=int, default=12_500_000, help="rays per evaluation realisation")
+    ap.add_argument("--sur-epochs", type=int, default=4, help="training epochs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for HBM counters)")
     return ap.parse_args()
@@ -103,7 +120,6 @@ def main():
 
     ray_steps = p.ray_steps
     value = ray_steps / (ms_per_step / 1e3)
-    achieved = BYTES_PER_RAY_STEP * ray_steps / (trace_ms / 1e3) / 1e9
 
     if args.calib_copy:
         calibration_copy()
@@ -142,30 +158,17 @@ def main():
         },
         "rays_per_s": cfg.n_rays / (ms_per_step / 1e3),
         "stage_ms": stages,
-        "roofline": {
-            "kernel": "trace_rk4_kernel",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(cfg),
-            "traffic_source": "profiles/trace_rk4_512_1e7_hbm_traffic.md (FETCH_SIZE x 2 + WRITE_SIZE x 1, "
-                              "calibrated on a 1 GiB copy)",
-            "trace_ms": trace_ms,
-            "bytes_per_ray_step": BYTES_PER_RAY_STEP,
-            "compute": profiled_valu(cfg),
-        },
+        "roofline": trace_roofline(cfg, trace_ms, "profiles/trace_rk4_512_1e7_hbm_traffic.md"),
         "cpu_baseline": None,
     }
 
     if not args.no_surrogate:
-        out["surrogate"] = surrogate_leg(p, cfg, args.sur_fields, args.sur_block, stages)
+        out["surrogate"] = surrogate_leg(cfg, args.sur_fields, args.sur_rays, args.sur_eval_fields,
+                                         args.sur_eval_rays, args.sur_epochs, stages)
 
     if base is not None:
-        log(f"cpu baseline: {base.P} processes, bundles of {args.cpu_bundle} rays, ~{args.cpu_budget:.0f} s")
         try:
-            out["cpu_baseline"] = cpu_baseline(base, p, cfg, args.cpu_bundle, args.cpu_budget)
+            out["cpu_baseline"] = cpu_baseline(base, p, cfg, args.cpu_bundle, args.cpu_budget, out["rays_per_s"])
         finally:
             base.close()
     print(json.dumps(out), flush=True)
@@ -220,8 +223,6 @@ def main_large(args):
     trace_ms = a.elapsed_time(b)
     if args.calib_copy:
         calibration_copy()
-    wave_steps = run.wave * cfg.steps_per_ray
-    achieved = BYTES_PER_RAY_STEP * wave_steps / (trace_ms / 1e3) / 1e9
     out = {
         "metric": "ray-steps/s at 1024^3, 1e8 rays in 1e7-ray waves; surrogate vs plain speedup at fixed error",
         "value": run.ray_steps / (ms_per_step / 1e3),
@@ -234,11 +235,7 @@ def main_large(args):
                                f"steps/ray), {cfg.pix}^2 histograms",
                    "grid": cfg.n, "rays": run.total, "wave": run.wave, "steps_per_ray": cfg.steps_per_ray},
         "rays_per_s": run.total / (ms_per_step / 1e3),
-        "roofline": {"kernel": "trace_rk4_kernel", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(run.p.cfg),
-                     "traffic_source": "profiles/trace_rk4_1024_1e7_hbm_traffic.md (FETCH_SIZE x 2 + WRITE_SIZE x 1, "
-                                       "calibrated on a 1 GiB copy)",
-                     "trace_ms": trace_ms, "bytes_per_ray_step": BYTES_PER_RAY_STEP},
+        "roofline": trace_roofline(run.p.cfg, trace_ms, "profiles/trace_rk4_1024_1e7_hbm_traffic.md"),
         "cpu_baseline": None,
     }
     if not args.no_surrogate:
@@ -255,7 +252,7 @@ def surrogate_large(run, cfg, plain_ms):
     import torch
     from turbulence_tracing_amd.pipeline import Config
     from turbulence_tracing_amd.surrogate import (Surrogate, block_fields, build_dataset, choose_block, errors,
-                                                  make_norm, normalise_targets, summarize)
+                                                  make_norm, normalise_targets, path_estimate, residual, summarize)
     p = run.p
     p.beam(100, n=run.wave, offset=0)
     p.sort()
@@ -266,11 +263,16 @@ def surrogate_large(run, cfg, plain_ms):
     k, kstats = choose_block(p.field4, p.grid, p.s0[:, :probe], rf0[:, :probe], ph0[:probe])
     log(f"surrogate: block size {k} ({kstats})")
     t0 = time.perf_counter()
-    feat, resid, _ = build_dataset(8, 200_000, Config(**{**cfg.__dict__, "pix": 64}), seed=60_000, k=k)
+    feat, resid, pt = build_dataset(8, 200_000, Config(**{**cfg.__dict__, "pix": 64}), seed=60_000, k=k)
     norm = make_norm(feat, resid)
     sur = Surrogate(norm, seed=1, k=k)
     sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=5e-4)
     del feat, resid
+    # error heads calibrated on a held-out realisation
+    pt.run(seed=61_001)
+    bc, fc = path_estimate(pt.field4, pt.grid, pt.s0, k=k)
+    cal = sur.calibrate(fc, residual(pt.rf, pt.phase, bc), 1e-6, 1e-6, margin=0.8)
+    del bc, fc, pt
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     blocks = block_fields(p.field4, p.grid, k) if k > 1 else None  # k = 1 reads the node field
@@ -300,7 +302,7 @@ def surrogate_large(run, cfg, plain_ms):
     um = lambda d: {a: round(b * 1e6, 5) for a, b in d.items()}
     res = {"speedup": plain_ms / sur_ms, "plain_ms": plain_ms, "surrogate_ms": sur_ms, "block_cells": k,
            "block_probe": {str(a): [round(x * 1e6, 4) for x in v] for a, v in kstats.items()},
-           "rays_fallback": nfb, "err_vs_plain_first_wave": {"pos_um": um(summarize(e[0])),
+           "rays_fallback": nfb, "calibration": cal, "err_vs_plain_first_wave": {"pos_um": um(summarize(e[0])),
                                                              "ang_urad": um(summarize(e[1]))},
            "training": {"fields": 8, "rays_per_field": 200_000, "seconds": round(train_s, 2)}}
     log(f"surrogate: {sur_ms:.0f} ms vs plain {plain_ms:.0f} ms (x{res['speedup']:.2f})")
@@ -327,98 +329,186 @@ def profiled_traffic(cfg):
     return None
 
 
-def profiled_valu(cfg):
-    """The trace kernel's VALU issue, measured by rocprofv3 (SQ_INSTS_VALU
-    over the kernel's cycles) for the 512^3 / 1e7-ray workload: the kernel
-    is bound by instruction issue, not by HBM (its measured traffic is a
-    few GB per launch)."""
-    if (cfg.n, cfg.n_rays, cfg.variant) != (512, 10_000_000, 0):
-        return None
-    path = os.path.join(ROOT, "profiles", "trace_rk4_512_1e7_valu.md")
-    out = {}
+VALU_PROFILES = {512: "trace_rk4_512_1e7_valu.md", 1024: "trace_rk4_1024_1e7_valu.md"}
+
+
+def profiled_valu_per_wave_step(cfg):
+    """VALU wave-instructions per wave-step (one RK4 step of 64 rays) of the
+    trace kernel at this grid size, measured by rocprofv3 (SQ_INSTS_VALU /
+    (SQ_WAVES x steps per ray)) on this binary and read from its committed
+    summary; None when there is none for this workload."""
+    name = VALU_PROFILES.get(cfg.n)
+    if name is None or cfg.variant != 0:
+        return None, None
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("trace_rk4_valu_wave_instr_per_step:"):
-                out["valu_wave_instr_per_step"] = int(line.split(":")[1])
-            elif line.startswith("trace_rk4_valu_frac:"):
-                out["valu_frac_of_issue_peak"] = float(line.split(":")[1])
+                return float(line.split(":")[1]), "profiles/" + name
     except OSError:
-        return None
-    out["source"] = "profiles/trace_rk4_512_1e7_valu.md"
-    return out or None
+        pass
+    return None, None
 
 
-def surrogate_leg(p, cfg, n_fields, k, stages):
-    """Surrogate + fallback against the plain tracer on the bench workload
-    (the realisation of the stage pass): the network is trained on rays the
-    tracer traces through n_fields other seeded realisations; the speedup is
-    plain trace time / surrogate time (block fields, path estimate, MLP,
-    apply, fallback) at the error the surrogate reaches against the plain
-    trace of the same rays."""
+def trace_roofline(cfg, trace_ms, traffic_source):
+    """The trace kernel against the roofline that binds it (VALU issue),
+    with its HBM traffic and algorithmic gather bytes beside it."""
+    import math
+    waves = math.ceil(cfg.n_rays / 64)
+    wave_steps = waves * cfg.steps_per_ray
+    per_step, src = profiled_valu_per_wave_step(cfg)
+    t = trace_ms / 1e3
+    traffic = profiled_traffic(cfg)
+    out = {"kernel": "trace_rk4_kernel", "bound": "valu", "unit": "TFLOP/s", "peak": FP32_PEAK_TFLOPS,
+           "achieved": None, "frac": None, "traffic": traffic, "trace_ms": trace_ms}
+    if per_step is not None:
+        instr = per_step * wave_steps
+        achieved = instr * 64 * 2 / t / 1e12
+        out.update(achieved=achieved, frac=achieved / FP32_PEAK_TFLOPS,
+                   valu_wave_instr_per_wave_step=per_step, valu_wave_instr_per_launch=instr,
+                   valu_source=src)
+    out["hbm"] = {"achieved_gbs": None if traffic is None else traffic / t / 1e9, "peak_gbs": HBM_PEAK_GBS,
+                  "frac": None if traffic is None else traffic / t / 1e9 / HBM_PEAK_GBS,
+                  "traffic_bytes": traffic,
+                  "traffic_source": traffic_source + " (FETCH_SIZE x 2 + WRITE_SIZE x 1, calibrated on a 1 GiB copy)"}
+    out["algorithmic_gather"] = {"bytes_per_ray_step": BYTES_PER_RAY_STEP,
+                                 "gbs": BYTES_PER_RAY_STEP * cfg.n_rays * cfg.steps_per_ray / t / 1e9,
+                                 "note": "trilinear-gather bytes, served from registers (the cell cache) and L1; "
+                                         "not an HBM rate"}
+    return out
+
+
+MLP_FLOP_PER_RAY = 2 * (48 * 256 + 256 * 256 + 256 * 256 + 256 * 8)  # the fused network's forward pass
+
+
+def surrogate_leg(cfg, n_fields, rays_per_field, eval_fields, eval_rays, epochs, stages):
+    """The learned-correction form at its stated scale: the network is
+    trained on rays the tracer traces through ``n_fields`` seeded 512^3
+    realisations (``rays_per_field`` each), its error heads are calibrated on
+    rays of one more held-out realisation, and it is evaluated against the
+    plain tracer on ``eval_fields`` further held-out realisations of
+    ``eval_rays`` rays each: per field the plain trace and the surrogate
+    trace (block fields, path estimate, network, correction, fallback on
+    flagged rays) of the same rays are timed, and the surrogate's errors are
+    taken against the plain trace.  The block size is the largest whose
+    bare estimate meets the error budget on 1e6 probe rays."""
     import numpy as np
     import torch
-    from turbulence_tracing_amd.pipeline import Config
-    from turbulence_tracing_amd.surrogate import (Surrogate, build_dataset, errors, make_norm, normalise_targets,
-                                                  path_estimate, block_fields, summarize)
+    from turbulence_tracing_amd.pipeline import Config, Pipeline
+    from turbulence_tracing_amd.surrogate import (Surrogate, build_dataset, choose_block, errors, make_norm,
+                                                  normalise_targets, path_estimate, residual)
+    bar = (1e-6, 1e-6)
+    stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
+    # block size from a probe realisation (held out)
+    pe = Pipeline(Config(**{**cfg.__dict__, "n_rays": eval_rays, "pix": 64}))
+    pe.run(seed=90_001)
+    probe = 1_000_000
+    k, kstats = choose_block(pe.field4, pe.grid, pe.s0[:, :probe], pe.rf[:, :probe], pe.phase[:probe])
+    log(f"surrogate: block size {k} ({ {a: [round(x * 1e6, 3) for x in v] for a, v in kstats.items()} } urad/um)")
+    # training set
     tcfg = Config(**{**cfg.__dict__, "pix": 64})
-    feat, resid, _ = build_dataset(n_fields, 100_000, tcfg, seed=50_000, k=k)
+    feat, resid, pt = build_dataset(n_fields, rays_per_field, tcfg, seed=50_000, k=k, log=log)
     norm = make_norm(feat, resid)
-    sur = Surrogate(norm, seed=1, k=k)
-    hist = sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=5e-4)
+    sur = Surrogate(norm, seed=1, k=k, tau=0.9)
+    hist = sur.train(feat, normalise_targets(resid, norm), epochs=epochs, batch=8192, lr=5e-4, log=log)
     del feat, resid
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
-    log(f"surrogate: trained on {n_fields} x 1e5 rays in {train_s:.1f} s")
-    stream = torch.cuda.current_stream()
-    thr = (1e-6, 1e-6)
+    log(f"surrogate: trained on {n_fields} x {rays_per_field} rays in {train_s:.1f} s")
+    # calibration on a held-out realisation (1e6 rays)
+    pe.run(seed=90_003)
+    bc, fc = path_estimate(pe.field4, pe.grid, pe.s0[:, :probe], k=k)
+    cal = sur.calibrate(fc, residual(pe.rf[:, :probe], pe.phase[:probe], bc), bar[0], bar[1], margin=0.8)
+    del bc, fc
+    log(f"surrogate: calibration {cal}")
 
-    def timed(fn, reps=3):
-        ms = []
-        for _ in range(reps):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            r = fn()
-            b.record(stream)
-            torch.cuda.synchronize()
-            ms.append(a.elapsed_time(b))
-        return float(np.mean(ms[1:] if reps > 1 else ms)), r
+    def timed(fn):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        r = fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b), r
 
-    plain_ms, _ = timed(p.trace)
+    def kth(e, q):
+        e = e[torch.isfinite(e)].double()
+        return float(torch.kthvalue(e.cpu(), max(1, int(np.ceil(q * e.numel()))))[0]) if e.numel() else 0.0
+
+    plain_ms = sur_ms = 0.0
+    n_rays = n_fb = n_over = 0
+    emax = [0.0, 0.0, 0.0]
+    p999 = [0.0, 0.0, 0.0]
+    sq = [0.0, 0.0, 0.0]
+    gains = None
     tim = {}
-    sur_ms, (rf, ph, flag) = timed(lambda: sur.trace(p.field4, p.grid, p.s0, t_pos=thr[0], t_ang=thr[1]))
-    sur.trace(p.field4, p.grid, p.s0, t_pos=thr[0], t_ang=thr[1], timings=tim)
-    e = errors(rf, ph, p.rf, p.phase)
-    # what the network gains over the estimate alone, on this held-out realisation
-    base, f2 = path_estimate(p.field4, p.grid, p.s0, k=k)
-    Y = sur.predict(f2)
-    rfn, phn, _, _, _ = sur.apply(base, Y, 1.0, 1.0)
-    eb = errors(torch.stack([base[0], base[1], base[2], base[3]]), base[4], p.rf, p.phase)
-    en = errors(rfn, phn, p.rf, p.phase)
-    rms = lambda v: float(v.double().pow(2).mean().sqrt())
-    um = lambda d: {a: round(b * 1e6, 5) for a, b in d.items()}
+    sur.trace(pe.field4, pe.grid, pe.s0)  # warm-up: buffers and packed weights
+    for f in range(eval_fields):
+        pe.run(seed=91_000 + 2 * f)
+        torch.cuda.synchronize()
+        pms, _ = timed(pe.trace)
+        sms, (rf, ph, flag) = timed(lambda: sur.trace(pe.field4, pe.grid, pe.s0, t_pos=bar[0], t_ang=bar[1]))
+        plain_ms += pms
+        sur_ms += sms
+        e = errors(rf, ph, pe.rf, pe.phase)
+        n_rays += pe.s0.shape[1]
+        n_fb += int(flag.sum())
+        n_over += int(((e[0] > bar[0]) | (e[1] > bar[1])).sum())
+        for i in range(3):
+            fin = e[i][torch.isfinite(e[i])]
+            emax[i] = max(emax[i], float(fin.max()))
+            p999[i] = max(p999[i], kth(e[i], 0.999))
+            sq[i] += float(fin.double().pow(2).sum())
+        if f == 0:
+            sur.trace(pe.field4, pe.grid, pe.s0, t_pos=bar[0], t_ang=bar[1], timings=tim)
+            # what the network gains over the path estimate alone, on this held-out field
+            base, f2 = path_estimate(pe.field4, pe.grid, pe.s0, k=k)
+            Y = sur.predict(f2)
+            rfn, phn, _, _, _ = sur.apply(base, Y, 1.0, 1.0)
+            eb = errors(torch.stack([base[0], base[1], base[2], base[3]]), base[4], pe.rf, pe.phase)
+            en = errors(rfn, phn, pe.rf, pe.phase)
+            rms = lambda v: float(v.double().pow(2).mean().sqrt())
+            gains = {"pos": rms(en[0]) / rms(eb[0]), "ang": rms(en[1]) / rms(eb[1]), "phase": rms(en[2]) / rms(eb[2])}
+            del base, f2, Y, rfn, phn, eb, en
+        log(f"surrogate field {f}: plain {pms:.1f} ms, surrogate {sms:.1f} ms, fallback {int(flag.sum())}")
+    um = lambda v: round(v * 1e6, 5)
+    mlp_ms = tim.get("mlp")
     res = {
         "speedup": plain_ms / sur_ms,
-        "plain_trace_ms": plain_ms,
-        "surrogate_ms": sur_ms,
-        "stage_ms": {a: round(b, 4) for a, b in tim.items() if a != "n_fallback"},
-        "rays_fallback": int(tim.get("n_fallback", 0)),
-        "thresholds": {"pos_m": thr[0], "ang_rad": thr[1]},
-        "err_vs_plain": {"pos_um": um(summarize(e[0])), "ang_urad": um(summarize(e[1])),
-                         "phase_mrad": {a: round(b * 1e3, 5) for a, b in summarize(e[2]).items()}},
-        "step_ms_with_surrogate": sum(v for a, v in stages.items() if a != "trace") + sur_ms,
-        "network_gain_rms": {"pos": rms(en[0]) / rms(eb[0]), "ang": rms(en[1]) / rms(eb[1]),
-                             "phase": rms(en[2]) / rms(eb[2])},
+        "plain_trace_ms": plain_ms, "surrogate_ms": sur_ms,
+        "rays": n_rays, "fields": eval_fields,
+        "rays_fallback": n_fb, "fallback_frac": n_fb / n_rays,
+        "rays_over_bar": n_over,
+        "thresholds": {"pos_m": bar[0], "ang_rad": bar[1]},
+        "err_vs_plain": {"pos_um": {"max": um(emax[0]), "p999": um(p999[0]), "rms": um(np.sqrt(sq[0] / n_rays))},
+                         "ang_urad": {"max": um(emax[1]), "p999": um(p999[1]), "rms": um(np.sqrt(sq[1] / n_rays))},
+                         "phase_mrad": {"max": round(emax[2] * 1e3, 5), "p999": round(p999[2] * 1e3, 5),
+                                        "rms": round(np.sqrt(sq[2] / n_rays) * 1e3, 5)}},
+        "stage_ms_first_field": {a: round(b, 4) for a, b in tim.items() if a not in ("n_fallback", "chunks")},
+        "network_gain_rms": gains,
         "block_cells": k,
-        "training": {"fields": n_fields, "rays_per_field": 100_000, "epochs": 6, "seconds": round(train_s, 2),
-                     "final_loss": [float(v) for v in hist[-1]]},
+        "block_probe": {str(a): [round(x * 1e6, 4) for x in v] for a, v in kstats.items()},
+        "calibration": cal,
+        "mlp_roofline": None if not mlp_ms else {
+            "kernel": "mlp_infer_kernel", "bound": "mfma", "unit": "TFLOP/s", "peak": BF16_PEAK_TFLOPS,
+            "achieved": MLP_FLOP_PER_RAY * eval_rays / (mlp_ms / 1e3) / 1e12,
+            "frac": MLP_FLOP_PER_RAY * eval_rays / (mlp_ms / 1e3) / 1e12 / BF16_PEAK_TFLOPS, "ms": mlp_ms},
+        "training": {"fields": n_fields, "rays_per_field": rays_per_field, "rays": n_fields * rays_per_field,
+                     "epochs": epochs, "seconds": round(train_s, 2), "final_loss": [float(v) for v in hist[-1]]},
+        "step_ms_with_surrogate": sum(v for a, v in stages.items() if a != "trace") + sur_ms / eval_fields,
     }
-    log(f"surrogate: {sur_ms:.2f} ms vs plain {plain_ms:.2f} ms (x{res['speedup']:.2f}); "
-        f"max err pos {res['err_vs_plain']['pos_um'].get('max')} um, ang {res['err_vs_plain']['ang_urad'].get('max')} urad")
+    log(f"surrogate: {sur_ms:.1f} ms vs plain {plain_ms:.1f} ms over {n_rays:.2e} rays (x{res['speedup']:.2f}); "
+        f"fallback {n_fb}; max err pos {res['err_vs_plain']['pos_um']['max']} um, "
+        f"ang {res['err_vs_plain']['ang_urad']['max']} urad")
+    del pe, pt
+    torch.cuda.empty_cache()
     return res
 
 
-def cpu_baseline(base, p, cfg, bundle, budget):
+def cpu_baseline(base, p, cfg, bundle, budget, gpu_rays_per_s):
+    """Both CPU modes on samples of the realisation the pipeline holds; each
+    mode's error is taken against the GPU trace of the same rays (itself
+    within 1e-6 cells / 1e-5 of the deflection of tight solve_ivp)."""
     import numpy as np
     import torch
     c = 299792458.0
@@ -430,21 +520,45 @@ def cpu_baseline(base, p, cfg, bundle, budget):
     del f4
     base.load_gradients(grads[0], grads[1], grads[2], x, x, x)
     del grads
-    s0 = p.s0_raw[:, :base.P * bundle * 64].cpu().numpy()
+    n_take = base.P * bundle * 64
+    s0 = p.s0_raw[:, :n_take].cpu().numpy()
+    # the GPU's exit rays of those rays (p.rf is in the sorted order p.order)
+    rf_raw = torch.empty_like(p.rf)
+    rf_raw[:, p.order.long()] = p.rf
+    rf_gpu = rf_raw[:, :n_take].cpu().numpy()
     torch.cuda.synchronize()
     log("cpu baseline: gradients in shared memory")
-    nray, wall, nfev = base.run_timed(s0, bundle, budget, log)
-    rays_per_s = nray / wall
+    launch = np.arctan(s0[[3, 4]] / s0[5])
+
+    def err(rf, n):
+        g = rf_gpu[:, :n]
+        defl = float(np.sqrt(np.mean((g[[1, 3]] - launch[:, :n]) ** 2)))
+        return {"pos_cells_max": float(np.abs(rf[[0, 2]] - g[[0, 2]]).max() / p.grid.dx),
+                "ang_max_of_rms_deflection": float(np.abs(rf[[1, 3]] - g[[1, 3]]).max() / defl)}
+
+    modes = {}
+    for mode, b in (("as_shipped", bundle), ("stated_tolerance", 1)):
+        log(f"cpu baseline ({mode}): {base.P} processes, {b} rays per process per round, ~{budget:.0f} s")
+        rf, nray, wall, nfev = base.run_timed(s0, b, budget, mode, log)
+        rps = nray / wall
+        modes[mode] = {"rays_per_s": rps, "rays": nray, "wall_s": round(wall, 2),
+                       "gpu_equivalent_ray_steps_per_s": rps * cfg.steps_per_ray,
+                       "own_ray_steps_per_s": nfev / 6 / wall, "own_steps_per_ray": nfev / 6 / max(nray, 1),
+                       "err_vs_gpu": err(rf, nray), "gpu_over_cpu_rays_per_s": gpu_rays_per_s / rps}
+    head = modes["stated_tolerance"]
     return {
-        "value": rays_per_s * cfg.steps_per_ray,
+        "value": head["gpu_equivalent_ray_steps_per_s"],
         "unit": "ray-steps/s",
-        "rays_per_s": rays_per_s,
+        "rays_per_s": head["rays_per_s"],
         "cores": base.P,
         "kind": "port",
-        "sample": f"{nray} rays of the same beam through the same {cfg.n}^3 field, in rounds of {base.P} "
-                  f"processes x {bundle} rays, one as-shipped solve_ivp RK45 (rtol 1e-3) per bundle; "
-                  f"ray-steps counted as rays x {cfg.steps_per_ray} (the GPU's steps per ray); wall {wall:.2f} s, "
-                  f"nfev {nfev}",
+        "sample": f"rays of the same beam through the same {cfg.n}^3 field on {base.P} processes "
+                  f"(os.cpu_count() = {base.cpu_count}, affinity {base.affinity}; at most {16} used, the host-core "
+                  f"share of one GPU): value = the stated-tolerance mode ({head['rays']} rays, one solve_ivp RK45 "
+                  f"per ray at rtol 1e-6, within 1e-3 of the RMS deflection of a tight solve), in ray-steps of the "
+                  f"GPU's problem (rays x {cfg.steps_per_ray}); the as-shipped mode ({modes['as_shipped']['rays']} "
+                  f"rays, one RK45 per {bundle}-ray bundle at rtol 1e-3) is in modes",
+        "modes": modes,
     }
 
 

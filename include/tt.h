@@ -138,6 +138,17 @@ int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *
                  double *rf, int64_t ld_rf, double *phase, double *sf, int64_t ld_sf,
                  int32_t *status, void *stream);
 
+/* tt_trace_rk4 over a ray list whose length is in device memory: the rays
+ * order[0 .. *n_dev) are traced (n_dev: device int32, read by the kernel;
+ * n_cap >= *n_dev bounds the launch), with no host synchronisation between
+ * the producer of the list and the trace.  No sf / status outputs.  No
+ * reference counterpart as a call: it serves the surrogate's fallback, which
+ * re-traces the rays tt_surrogate_apply flags (the full solve, PT:312-380,
+ * on those rays). */
+int tt_trace_rk4_counted(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
+                         int64_t ld_s0, int64_t n_cap, const int32_t *order, const int32_t *n_dev, double *rf,
+                         int64_t ld_rf, double *phase, void *stream);
+
 /* Ensembles: the rays of many realisations (uneven grids and ray counts)
  * traced in ONE persistent launch.  No reference counterpart as a call: it
  * replaces running ElectronCube.solve once per realisation, as the reference's
@@ -313,7 +324,8 @@ int tt_path_features(const float *field4, const float *GM4, int32_t k, const tt_
  * (radians) or the estimate is not finite; flag (optional) gets 0/1 per ray
  * and first + (index of each flagged ray) is appended to list (optional) with
  * *nflag (device int32, caller-zeroed) counting them, so batches of one ray
- * set can share one list. */
+ * set can share one list.  Each wave of 64 rays takes consecutive list slots
+ * for its flagged rays, in ray order. */
 int tt_surrogate_apply(const double *base, int64_t ldb, const float *Y, int64_t ldy, int64_t n, const float *scales,
                        const double *thresholds, double *rf, int64_t ldr, double *phase, int32_t *flag,
                        int32_t *list, int32_t *nflag, int64_t first, void *stream);
@@ -359,12 +371,14 @@ int tt_gelu_backward(const float *dH_f32, const void *dH_bf16, const float *Zpre
 /* db[c] += sum over rows of dZ[r][c] (bf16 input, fp32 accumulate) */
 int tt_bias_grad(const void *dZ_bf16, int64_t rows, int32_t cols, int64_t ld, float *db, void *stream);
 /* Surrogate loss: weighted squared error on columns [0, n_reg) of Y against
- * T, plus L1 error heads (column n_reg + h predicts max |T - Y| over columns
- * [err_lo[h], err_hi[h])).  Writes the gradient / rows to dY (bf16, ld ldd,
- * zero up to n_pad) and ADDS the two loss sums to loss_out (2 doubles). */
+ * T, plus pinball-loss error heads: column n_reg + h learns the tau-quantile
+ * (0 < tau < 1) of the corrected error max |T - Y| over columns
+ * [err_lo[h], err_hi[h]), i.e. a bound that tau of the rays stay under.
+ * Writes the gradient / rows to dY (bf16, ld ldd, zero up to n_pad) and ADDS
+ * the two loss sums to loss_out (2 doubles). */
 int tt_surrogate_loss(const float *Y, int64_t ldy, const float *T, int64_t ldt, void *dY_bf16, int64_t ldd,
                       int64_t rows, int32_t n_reg, const float *w_reg, int32_t n_err, const int32_t *err_lo,
-                      const int32_t *err_hi, int32_t n_pad, double *loss_out, void *stream);
+                      const int32_t *err_hi, double tau, int32_t n_pad, double *loss_out, void *stream);
 /* Adam (bias-corrected, step >= 1) on fp32 master weights; refreshes the
  * bf16 copy w_bf16 when non-NULL. */
 int tt_adam(float *w, const float *g, float *m, float *v, void *w_bf16, int64_t n, double lr, double b1, double b2,

@@ -118,7 +118,7 @@ def path_estimate(grid, s0, k=1, moments=True):
         inside = (mx >= 0) & (mx <= nx - 1) & (my >= 0) & (my <= ny - 1)
         g, mval, qx, qy, ix, iy = sample(mx, my, inside)
         sp = np.sqrt(U * U + V * V + W * W) / W
-        iWm = 1.0 / (W + hm * g[2] / W)
+        iWm = 1.0 / (W + 0.5 * g[2] / W)  # W at the middle: g[2] is the whole step's change
         Un, Vn, Wn = U + g[0] * iWm, V + g[1] * iWm, W + g[2] * iWm
         pxn = mx + hm * sx * Un / Wn - sx * mval[0] / Wn
         pyn = my + hm * sy * Vn / Wn - sy * mval[1] / Wn

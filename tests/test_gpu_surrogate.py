@@ -238,20 +238,28 @@ def test_empty_ray_set_through_the_surrogate(lib):
     assert tuple(rf.shape) == (4, 0) and ph.numel() == 0 and flag.numel() == 0
 
 
-def test_surrogate_gain_and_fallback(lib):
-    """Trained on rays the tracer generates through 32 seeded realisations,
-    the network corrects the block path estimate on a held-out realisation:
-    it removes most of the phase residual and does not degrade positions or
-    angles.  Flagged rays are re-traced by the full tracer, bit for bit; with
-    zero thresholds every ray falls back and the result is the tracer's."""
-    from turbulence_tracing_amd.pipeline import Config, Pipeline
-    from turbulence_tracing_amd.surrogate import (Surrogate, build_dataset, errors, make_norm, normalise_targets,
-                                                  path_estimate)
+def _train_128(k, seed=100, fields=32, rays=50000, tau=0.9):
+    from turbulence_tracing_amd.pipeline import Config
+    from turbulence_tracing_amd.surrogate import Surrogate, build_dataset, make_norm, normalise_targets
     cfg = Config(n=128, pix=64)
-    feat, resid, _ = build_dataset(32, 50000, cfg, seed=100, k=8)
+    feat, resid, _ = build_dataset(fields, rays, cfg, seed=seed, k=k)
     norm = make_norm(feat, resid)
-    sur = Surrogate(norm, seed=1, k=8)
+    sur = Surrogate(norm, seed=1, k=k, tau=tau)
     sur.train(feat, normalise_targets(resid, norm), epochs=6, batch=4096, lr=5e-4)
+    return sur
+
+
+def test_surrogate_gain_and_fallback(lib):
+    """Trained on rays the tracer generates through 32 seeded 128^3
+    realisations, the network corrects the block path estimate (k = 8) on a
+    held-out realisation: it cuts the RMS error of positions, angles and
+    phase (measured on MI355X: x0.80, x0.70, x0.22).  Flagged rays are
+    re-traced by the full tracer, bit for bit; with zero thresholds every ray
+    falls back and the result is the tracer's."""
+    from turbulence_tracing_amd.pipeline import Config, Pipeline
+    from turbulence_tracing_amd.surrogate import errors, path_estimate
+    sur = _train_128(8)
+    norm = sur.norm
     p = Pipeline(Config(n=128, pix=64, n_rays=200000))
     p.run(seed=999)
     base, f2 = path_estimate(p.field4, p.grid, p.s0, k=8)
@@ -262,14 +270,14 @@ def test_surrogate_gain_and_fallback(lib):
     rms = lambda e: float(e.double().pow(2).mean().sqrt())
     gain = [rms(e_mlp[i]) / rms(e_base[i]) for i in range(3)]
     print(f"network / estimate error RMS: position {gain[0]:.3f} angle {gain[1]:.3f} phase {gain[2]:.3f}")
-    assert gain[2] < 0.5
-    assert gain[0] < 1.05 and gain[1] < 1.05
+    assert gain[0] < 0.9 and gain[1] < 0.85 and gain[2] < 0.4, gain
     # an angle threshold at the 90th percentile of the network's own error
     # bound flags about a tenth of the rays; those are exactly the tracer's
     t_ang = float(torch.quantile((Y[:100000, 6].abs() * norm.e_ang).double(), 0.9))
     rf3, ph3, flag3 = sur.trace(p.field4, p.grid, p.s0, t_pos=1.0, t_ang=t_ang)
     fl = flag3.bool()
     assert 0 < int(fl.sum()) < p.s0.shape[1]
+    assert int(sur.n_fallback) == int(fl.sum())
     np.testing.assert_array_equal(rf3[:, fl].cpu().numpy(), p.rf[:, fl].cpu().numpy())
     np.testing.assert_array_equal(ph3[fl].cpu().numpy(), p.phase[fl].cpu().numpy())
     # the rays cut into batches on two streams: every ray's result is the same
@@ -281,3 +289,161 @@ def test_surrogate_gain_and_fallback(lib):
     assert int(flag2.sum()) == p.s0.shape[1]
     np.testing.assert_array_equal(rf2.cpu().numpy(), p.rf.cpu().numpy())
     np.testing.assert_array_equal(ph2.cpu().numpy(), p.phase.cpu().numpy())
+
+
+def test_surrogate_calibrated_fallback_on_harder_realisation(lib):
+    """A harder setting, blocks of 32 cells at 128^3, where the bare
+    estimate's angle error passes 1 urad on some rays.  The error heads
+    (pinball loss, quantile 0.9) are calibrated on one held-out realisation
+    so that every ray there whose corrected error exceeds 0.8 urad is
+    flagged.  On another held-out realisation every ray whose corrected error
+    exceeds the 1 um / 1 urad bar is flagged and re-traced, the surrogate's
+    result is then within the bar everywhere, and the flagged rays carry
+    larger errors than the rest."""
+    from turbulence_tracing_amd.pipeline import Config, Pipeline
+    from turbulence_tracing_amd.surrogate import errors, path_estimate, residual
+    k = 32
+    sur = _train_128(k)
+    pc = Pipeline(Config(n=128, pix=64, n_rays=200000))
+    pc.run(seed=998)
+    bc, fc = path_estimate(pc.field4, pc.grid, pc.s0, k=k)
+    cal = sur.calibrate(fc, residual(pc.rf, pc.phase, bc), 1e-6, 1e-6, margin=0.8)
+    print("calibration:", cal)
+    assert cal["ang"]["n_over_margin"] > 0 and cal["ang"]["max_unflagged"] <= 0.8e-6
+    p = Pipeline(Config(n=128, pix=64, n_rays=200000))
+    p.run(seed=999)
+    base, f2 = path_estimate(p.field4, p.grid, p.s0, k=k)
+    eb = errors(torch.stack([base[0], base[1], base[2], base[3]]), base[4], p.rf, p.phase)
+    assert int((eb[1] > 1e-6).sum()) > 0  # the bare estimate breaks the bar here
+    Y = sur.predict(f2)
+    rfn, phn, _, _, _ = sur.apply(base, Y, 1.0, 1.0)
+    en = errors(rfn, phn, p.rf, p.phase)
+    rf, ph, flag = sur.trace(p.field4, p.grid, p.s0, t_pos=1e-6, t_ang=1e-6)
+    fl = flag.bool()
+    over = (en[0] > 1e-6) | (en[1] > 1e-6)
+    print(f"bare over bar {int((eb[1] > 1e-6).sum())}, corrected over bar {int(over.sum())}, "
+          f"flagged {float(fl.double().mean()):.3f}")
+    assert bool(fl[over].all())
+    ea = errors(rf, ph, p.rf, p.phase)
+    assert float(ea[0].max()) <= 1e-6 and float(ea[1].max()) <= 1e-6
+    np.testing.assert_array_equal(rf[:, fl].cpu().numpy(), p.rf[:, fl].cpu().numpy())
+    if bool((~fl).any()):
+        assert float(en[1][fl].double().mean()) > float(en[1][~fl].double().mean())
+
+
+def test_surrogate_loss_pinball_matches_numpy(lib):
+    """tt_surrogate_loss: weighted squared error on the residual columns and
+    the pinball loss at quantile tau on the error heads, whose target is max
+    |T - Y| over each head's columns; gradients / rows in bf16."""
+    from turbulence_tracing_amd import _lib
+    rs = np.random.RandomState(8)
+    rows, n_reg, n_pad, tau = 1000, 5, 8, 0.9
+    Y = rs.randn(rows, n_pad).astype(np.float32)
+    T = rs.randn(rows, n_reg).astype(np.float32)
+    w = np.array([1, 1, 1, 1, 0.5], np.float32)
+    lo, hi = np.array([0, 2], np.int32), np.array([2, 4], np.int32)
+    Yd, Td, wd = (torch.as_tensor(a, device=DEV) for a in (Y, T, w))
+    lod, hid = torch.as_tensor(lo, device=DEV), torch.as_tensor(hi, device=DEV)
+    dY = torch.empty((rows, n_pad), dtype=torch.bfloat16, device=DEV)
+    loss = torch.zeros(2, dtype=torch.float64, device=DEV)
+    lib.tt_surrogate_loss(Yd.data_ptr(), n_pad, Td.data_ptr(), n_reg, dY.data_ptr(), n_pad, rows, n_reg, wd.data_ptr(),
+                          2, lod.data_ptr(), hid.data_ptr(), tau, n_pad, loss.data_ptr(), _lib.stream_ptr())
+    torch.cuda.synchronize()
+    e = Y[:, :n_reg].astype(np.float64) - T
+    want_d = np.zeros((rows, n_pad))
+    want_d[:, :n_reg] = 2 * w * e / rows
+    pin = 0.0
+    for h in range(2):
+        tgt = np.abs(T[:, lo[h]:hi[h]] - Y[:, lo[h]:hi[h]]).max(1)
+        d = Y[:, n_reg + h].astype(np.float64) - tgt
+        pin += np.sum(np.where(d > 0, (1 - tau) * d, -tau * d))
+        want_d[:, n_reg + h] = np.where(d > 0, 1 - tau, -tau) / rows
+    got = dY.float().cpu().numpy()
+    np.testing.assert_allclose(got, want_d, rtol=1e-2, atol=1e-2 / rows)
+    l = loss.cpu().numpy()
+    np.testing.assert_allclose(l[0], np.sum(w * e ** 2), rtol=1e-5)
+    np.testing.assert_allclose(l[1], pin, rtol=1e-5)
+
+
+def test_counted_trace_and_wave_ordered_list(lib):
+    """tt_surrogate_apply lists the flagged rays with one slot range per
+    wave (ray order inside each wave); tt_trace_rk4_counted traces exactly
+    the first *n_dev listed rays, bit for bit as tt_trace_rk4 on that list,
+    and leaves the other rays' outputs untouched."""
+    import ctypes
+    from turbulence_tracing_amd import _lib
+    from turbulence_tracing_amd.surrogate import MLP, Norm, Surrogate, path_estimate
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    x, ne = small_cube(n=33)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne)
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(20000, 2e-3, 50e-6, seed=7)
+    n = s0.shape[1]
+    sur = Surrogate(Norm(np.zeros(48, np.float32), np.ones(48, np.float32), 1e-6, 1e-6, 1e-3), mlp=MLP(seed=2), k=4)
+    base, feat = path_estimate(cube.field4, cube.grid, s0, k=4)
+    Y = sur.predict(feat)
+    t_ang = float(torch.quantile((Y[:, 6].abs() * sur.norm.e_ang).double(), 0.7))
+    rf, ph, flag, lst, cnt = sur.apply(base, Y, 1.0, t_ang)
+    torch.cuda.synchronize()
+    m = int(cnt.item())
+    fl = flag.bool().cpu().numpy()
+    L = lst[:m].cpu().numpy()
+    assert m == fl.sum() and np.array_equal(np.sort(L), np.flatnonzero(fl))
+    # each wave's flagged rays sit in consecutive slots, in ray order
+    waves = L // 64
+    for wv in np.unique(waves):
+        pos = np.flatnonzero(waves == wv)
+        assert np.array_equal(pos, np.arange(pos[0], pos[0] + pos.size))
+        assert np.all(np.diff(L[pos]) > 0)
+    opts = _lib.TraceOpts(0.5, 0.0, 0)
+    want_rf = torch.full((4, n), 7.0, dtype=torch.float64, device=DEV)
+    want_ph = torch.full((n,), 7.0, dtype=torch.float64, device=DEV)
+    lib.tt_trace_rk4(cube.field4.data_ptr(), ctypes.byref(cube.grid), ctypes.byref(opts), s0.data_ptr(), n, m,
+                     lst.data_ptr(), want_rf.data_ptr(), n, want_ph.data_ptr(), None, n, None, _lib.stream_ptr())
+    got_rf = torch.full((4, n), 7.0, dtype=torch.float64, device=DEV)
+    got_ph = torch.full((n,), 7.0, dtype=torch.float64, device=DEV)
+    lib.tt_trace_rk4_counted(cube.field4.data_ptr(), ctypes.byref(cube.grid), ctypes.byref(opts), s0.data_ptr(), n,
+                             n, lst.data_ptr(), cnt.data_ptr(), got_rf.data_ptr(), n, got_ph.data_ptr(),
+                             _lib.stream_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(got_rf, want_rf) and torch.equal(got_ph, want_ph)
+    assert bool((got_ph[torch.as_tensor(~fl, device=DEV)] == 7.0).all())
+
+
+def test_surrogate_trace_graph_capture_replays(lib):
+    """Surrogate.trace waits on nothing from the host, so it can be
+    captured in a graph: three replays, with eager work and
+    synchronisations between them, each equal the eager result."""
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    from turbulence_tracing_amd.surrogate import MLP, Norm, Surrogate, block_fields, path_estimate
+    x, ne = small_cube(n=33)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne)
+    cube.calc_dndr(lwl=1064e-9)
+    s0 = cube.init_beam(30000, 2e-3, 50e-6, seed=11)
+    sur = Surrogate(Norm(np.zeros(48, np.float32), np.ones(48, np.float32), 1e-6, 1e-6, 1e-3), mlp=MLP(seed=5), k=4)
+    blocks = block_fields(cube.field4, cube.grid, 4)
+    # a threshold that flags about 30% of the rays of this untrained network
+    _, feat = path_estimate(cube.field4, cube.grid, s0, k=4, blocks=blocks)
+    t_ang = float(torch.quantile((sur.predict(feat)[:, 6].abs() * sur.norm.e_ang).double(), 0.7))
+    rf0, ph0, fl0 = sur.trace(cube.field4, cube.grid, s0, t_pos=1.0, t_ang=t_ang, blocks=blocks)
+    torch.cuda.synchronize()
+    assert 0 < int(fl0.sum()) < s0.shape[1]
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):  # warm-up on a side stream, as capture requires
+        sur.trace(cube.field4, cube.grid, s0, t_pos=1.0, t_ang=t_ang, blocks=blocks)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        rf_g, ph_g, fl_g = sur.trace(cube.field4, cube.grid, s0, t_pos=1.0, t_ang=t_ang, blocks=blocks)
+    for _ in range(3):
+        junk = torch.randn(1 << 20, device=DEV).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(fl_g, fl0)
+        assert torch.equal(rf_g.nan_to_num(7.0), rf0.nan_to_num(7.0)) and torch.equal(ph_g, ph0)
+        del junk

@@ -152,10 +152,10 @@ def test_block_position_moment_exact_for_z_linear_kick(k):
 def test_block_kick_moment_exact_for_tilted_rays():
     """dn/dx = a x z: along a tilted ray the field is bilinear in the ray's
     x and z, and the centre value G plus v . dM/dx is its exact block
-    integral.  What remains at 16-cell blocks is second order (the change of
-    W inside a block, which here is comparable with the small deflection):
-    within 2e-2 of the deflection, against about 25% without the moment
-    term."""
+    integral.  What remains at 16-cell blocks is second order (the ray's
+    bending and W's change inside a block, which here are comparable with
+    the small deflection): within 1.2e-2 of the deflection, against about
+    25% without the moment term."""
     n = 33
     x = np.linspace(-5e-3, 5e-3, n)
     XX, _, ZZ = np.meshgrid(x, x, x, indexing="ij")
@@ -167,7 +167,7 @@ def test_block_kick_moment_exact_for_tilted_rays():
     plain, _ = S.path_estimate(g, s0, k=16, moments=False)
     defl = np.abs(rf[1] - np.arctan(s0[3] / s0[5])).max()
     err = np.abs(base[1] - rf[1]).max()
-    assert err < 2e-2 * defl
+    assert err < 1.2e-2 * defl
     assert np.abs(plain[1] - rf[1]).max() > 10 * err
 
 
@@ -188,3 +188,24 @@ def test_path_estimate_close_to_rk4_turbulent(k):
     dx = 10e-3 / (n - 1)
     assert np.abs(base[[0, 2]] - rf[[0, 2]]).max() < 1e-3 * max(1, k * k / 4) * dx
     assert np.abs(base[4] - ph).max() < 1e-4 * max(1, k * k / 4) * np.abs(ph).max()
+
+
+@pytest.mark.parametrize("k", [4, 16])
+def test_block_kick_uses_mid_block_w(k):
+    """Constant gradients in x and in z: W changes linearly in z (dW/dZ =
+    g_z / W), so the exact kick is g_x (W_out - W_in) / g_z, which the block
+    kick G_x / W_mid reproduces to second order in W's change when W_mid =
+    W + G_z / 2 W (G_z being the whole block's change): within 1e-4 of the
+    deflection here.  Taking W's change over half a block as k / 2 times the
+    block's change is off by 4e-3 of it at k = 4 and 9e-2 at k = 16."""
+    n = 33
+    x = np.linspace(-5e-3, 5e-3, n)
+    XX, _, ZZ = np.meshgrid(x, x, x, indexing="ij")
+    c = cube_of(1e25 * (1 + 0.5 * XX / 5e-3 + 2.0 * ZZ / 5e-3), n)
+    s0 = c.init_beam(20, 1e-3, 0.0, rng=np.random.RandomState(4))
+    g = T.NodeGrid(c)
+    rf, ph, _, _ = T.zmarch_rk4(g, s0, h=0.125)
+    base, _ = S.path_estimate(g, s0, k=k)
+    defl = np.abs(rf[1]).max()
+    err = np.abs(base[1] - rf[1]).max()
+    assert err < 1e-4 * defl, err / defl

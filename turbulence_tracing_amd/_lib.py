@@ -92,6 +92,7 @@ _SIGNATURES = {
     "tt_power_spectrum": [P, P, I32, I32, D, D, P, P, P, P],
     "tt_grad_stencil": [P, I32, I64, I64, I64, C.POINTER(Grid), D, D, D, D, P, I64, P, P],
     "tt_trace_rk4": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, I64, P, P, I64, P, P],
+    "tt_trace_rk4_counted": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, P, I64, P, P],
     "tt_sample_field": [P, C.POINTER(Grid), P, I64, I64, P, I64, P],
     "tt_beam_sample": [I64, D, D, D, I32, U64, U64, P, I64, P],
     "tt_ray_sort_scratch_bytes": [C.POINTER(Grid), I32, SIZE_P],
@@ -106,7 +107,7 @@ _SIGNATURES = {
     "tt_gemm_bf16": [C.POINTER(GemmArgs), P],
     "tt_gelu_backward": [P, P, P, P, I64, P],
     "tt_bias_grad": [P, I64, I32, I64, P, P],
-    "tt_surrogate_loss": [P, I64, P, I64, P, I64, I64, I32, P, I32, P, P, I32, P, P],
+    "tt_surrogate_loss": [P, I64, P, I64, P, I64, I64, I32, P, I32, P, P, D, I32, P, P],
     "tt_adam": [P, P, P, P, P, I64, D, D, D, D, I64, D, P],
     "tt_batch_workspace_bytes": [C.POINTER(BatchItem), I32, I32, C.POINTER(C.c_size_t)],
     "tt_batch_prepare": [C.POINTER(BatchItem), I32, D, I32, P, C.c_size_t, C.POINTER(BatchInfo)],

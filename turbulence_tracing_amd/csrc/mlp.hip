@@ -185,6 +185,7 @@ struct LossArgs {
     float w[8];
     int err_lo[4], err_hi[4];  // error head j predicts the max |T - Y| over columns [lo, hi)
     double *loss_out;
+    float tau;
 };
 
 __global__ void loss_kernel(LossArgs a) {
@@ -200,12 +201,14 @@ __global__ void loss_kernel(LossArgs a) {
             s2 += (double)(a.w[j] * e * e);
             d[j] = (__bf16)(2.0f * a.w[j] * e * inv);
         }
+        // pinball loss at quantile tau: the head learns the tau-quantile of
+        // the corrected error max |T - Y| over its columns
         for (int h = 0; h < a.n_err; ++h) {
             float target = 0.f;
             for (int j = a.err_lo[h]; j < a.err_hi[h]; ++j) target = fmaxf(target, fabsf(t[j] - y[j]));
             const float e = y[a.n_reg + h] - target;
-            s1 += (double)fabsf(e);
-            d[a.n_reg + h] = (__bf16)((e > 0.f ? 1.0f : (e < 0.f ? -1.0f : 0.0f)) * inv);
+            s1 += (double)(e > 0.f ? (1.0f - a.tau) * e : -a.tau * e);
+            d[a.n_reg + h] = (__bf16)((e > 0.f ? 1.0f - a.tau : (e < 0.f ? -a.tau : 0.0f)) * inv);
         }
         for (int j = a.n_reg + a.n_err; j < a.n_pad; ++j) d[j] = (__bf16)0.0f;
     }
@@ -468,9 +471,10 @@ extern "C" int tt_bias_grad(const void *dZ_bf16, int64_t rows, int32_t cols, int
 
 extern "C" int tt_surrogate_loss(const float *Y, int64_t ldy, const float *T, int64_t ldt, void *dY_bf16,
                                  int64_t ldd, int64_t rows, int32_t n_reg, const float *w_reg, int32_t n_err,
-                                 const int32_t *err_lo, const int32_t *err_hi, int32_t n_pad, double *loss_out,
+                                 const int32_t *err_lo, const int32_t *err_hi, double tau, int32_t n_pad, double *loss_out,
                                  void *stream) {
     TT_REQUIRE(Y && T && dY_bf16 && loss_out && w_reg, "tt_surrogate_loss: null pointer");
+    TT_REQUIRE(tau > 0.0 && tau < 1.0, "tt_surrogate_loss: tau must be in (0, 1)");
     TT_REQUIRE(n_reg >= 1 && n_reg <= 8 && n_err >= 0 && n_err <= 4 && n_pad >= n_reg + n_err && n_pad <= ldd,
                "tt_surrogate_loss: bad column counts");
     TT_REQUIRE(ldy >= n_reg + n_err && ldt >= n_reg, "tt_surrogate_loss: bad leading dimensions");
@@ -493,6 +497,7 @@ extern "C" int tt_surrogate_loss(const float *Y, int64_t ldy, const float *T, in
         a.err_hi[h] = err_hi[h];
     }
     a.loss_out = loss_out;
+    a.tau = (float)tau;
     if (rows == 0) return TT_OK;
     hipLaunchKernelGGL(loss_kernel, dim3((unsigned)tt::ceil_div(rows, 256)), dim3(256), 0, tt::as_stream(stream), a);
     TT_LAUNCH_CHECK("tt_surrogate_loss launch");

@@ -113,7 +113,8 @@ __device__ __forceinline__ Mid half_drift(const PathArgs &a, const PathState &S,
 __device__ __forceinline__ void kick_drift(const PathArgs &a, const Mid &m, float4 g, float w, float mvx, float mvy,
                                            int c0, int ncell, PathState &S, float *f) {
     const float hm = 0.5f * w;
-    const float iWm = __builtin_amdgcn_rcpf(fmaf(hm * g.z, m.iW, m.W));
+    // W at the step's middle: g.z is the whole step's change of W (times W)
+    const float iWm = __builtin_amdgcn_rcpf(fmaf(0.5f * g.z, m.iW, m.W));
     const float sp = __builtin_amdgcn_sqrtf(fmaf(S.U, S.U, fmaf(S.V, S.V, m.W * m.W))) * m.iW;
     S.ph += (double)(sp * g.w);
     S.U = fmaf(g.x, iWm, S.U);
@@ -430,7 +431,7 @@ __global__ void apply_kernel(const double *base, int64_t ldb, const float *Y, in
                              double *rf, int64_t ldr, double *phase, int32_t *flag, int32_t *list, int32_t *nflag,
                              int64_t first) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n) return;  // the rest of the wave stays converged below
     const float *y = Y + i * ldy;
     const double x = base[i] + (double)(y[0] * s_pos);
     const double th = base[ldb + i] + (double)(y[2] * s_ang);
@@ -445,7 +446,20 @@ __global__ void apply_kernel(const double *base, int64_t ldb, const float *Y, in
     const double ep = fabs((double)(y[5] * e_pos)), ea = fabs((double)(y[6] * e_ang));
     const bool bad = !(ep <= t_pos && ea <= t_ang) || isnan(x) || isnan(th) || isnan(yy) || isnan(ph);
     if (flag) flag[i] = bad ? 1 : 0;
-    if (bad && list) list[atomicAdd(nflag, 1)] = (int32_t)(first + i);
+    if (list) {
+        // one atomic per wave: the wave's flagged rays take consecutive
+        // slots in lane order, so the list keeps the rays' entry-column
+        // order within each wave (the fallback trace stays coherent)
+        const uint64_t m = __ballot(bad);
+        if (m) {
+            const int lane = (int)__lane_id();
+            const int leader = __ffsll((unsigned long long)m) - 1;
+            int slot = 0;
+            if (lane == leader) slot = atomicAdd(nflag, (int)__popcll(m));
+            slot = __shfl(slot, leader, 64);
+            if (bad) list[slot + (int)__popcll(m & ((1ull << lane) - 1ull))] = (int32_t)(first + i);
+        }
+    }
 }
 
 }  // namespace

@@ -46,6 +46,7 @@ struct TraceArgs {
     double *sf;
     int64_t ld_sf;
     int32_t *status;
+    const int32_t *n_dev;  // optional (device): trace only launch indices below *n_dev
 };
 
 // qx, qy: (x, y) displacement from the entry point in units of dz (the
@@ -558,6 +559,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M2 ? 5 : 1)
     const int b = tt::xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int64_t t = (int64_t)b * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
+    if (a.n_dev && t >= (int64_t)*a.n_dev) return;
     trace_ray<CACHED, M2, TAU>(a, t);
 }
 
@@ -652,9 +654,10 @@ extern "C" int tt_sample_field(const float *field4, const tt_grid *grid, const d
     return TT_OK;
 }
 
-extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
-                            int64_t ld_s0, int64_t n, const int32_t *order, double *rf, int64_t ld_rf,
-                            double *phase, double *sf, int64_t ld_sf, int32_t *status, void *stream) {
+namespace {
+int trace_launch(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0, int64_t ld_s0,
+                 int64_t n, const int32_t *order, const int32_t *n_dev, double *rf, int64_t ld_rf, double *phase,
+                 double *sf, int64_t ld_sf, int32_t *status, void *stream) {
     TT_REQUIRE(grid && opts, "tt_trace_rk4: null grid or options");
     TT_REQUIRE(n >= 0, "tt_trace_rk4: n < 0");
     if (n == 0) return TT_OK;  // an empty ray set: nothing is read or written
@@ -680,6 +683,7 @@ extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_t
     a.sf = sf;
     a.ld_sf = ld_sf;
     a.status = status;
+    a.n_dev = n_dev;
     const int block = 256;
     TT_REQUIRE(opts->variant == 0 || opts->variant == 1, "tt_trace_rk4: variant must be 0 or 1");
     const dim3 nblk((unsigned)tt::ceil_div(n, block));
@@ -698,6 +702,22 @@ extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_t
         hipLaunchKernelGGL((trace_rk4_kernel<true, false, false>), nblk, dim3(block), 0, st, a);
     TT_LAUNCH_CHECK("tt_trace_rk4 launch");
     return TT_OK;
+}
+}  // namespace
+
+extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
+                            int64_t ld_s0, int64_t n, const int32_t *order, double *rf, int64_t ld_rf,
+                            double *phase, double *sf, int64_t ld_sf, int32_t *status, void *stream) {
+    return trace_launch(field4, grid, opts, s0, ld_s0, n, order, nullptr, rf, ld_rf, phase, sf, ld_sf, status,
+                        stream);
+}
+
+extern "C" int tt_trace_rk4_counted(const float *field4, const tt_grid *grid, const tt_trace_opts *opts,
+                                    const double *s0, int64_t ld_s0, int64_t n_cap, const int32_t *order,
+                                    const int32_t *n_dev, double *rf, int64_t ld_rf, double *phase, void *stream) {
+    TT_REQUIRE(n_cap == 0 || (order && n_dev), "tt_trace_rk4_counted: order and n_dev are required");
+    return trace_launch(field4, grid, opts, s0, ld_s0, n_cap, order, n_dev, rf, ld_rf, phase, nullptr, ld_rf, nullptr,
+                        stream);
 }
 
 namespace {

@@ -12,9 +12,15 @@ Not part of the reference (which has no surrogate).  The pieces:
     training and one fused kernel for inference) predicts the residual
     (full trace - estimate) for x, y, theta, phi, phase and two error
     bounds (position, angle);
-  * rays whose predicted error bound exceeds the threshold (1 um / 1 urad by
-    default), or whose estimate is not finite, are re-traced with
-    ``tt_trace_rk4``.
+  * the error heads are trained with a pinball loss, so each learns a
+    quantile of the corrected error given the features (an upper bound that
+    most rays stay under); ``Surrogate.calibrate`` then scales them on
+    held-out traced rays so that every held-out ray whose corrected error
+    exceeds a margin of the bar is flagged;
+  * rays whose scaled bound exceeds the threshold (1 um / 1 urad by
+    default), or whose estimate is not finite, are re-traced with the full
+    tracer (``tt_trace_rk4_counted``, which reads the number of flagged rays
+    from device memory, so the surrogate trace never waits on the host).
 Training data are rays the tracer itself generates through seeded fields.
 """
 import ctypes as C
@@ -196,13 +202,13 @@ class Norm:
     s_pos: float  # metres per unit of the x/y residual outputs
     s_ang: float  # radians per unit of the angle outputs
     s_ph: float   # radians per unit of the phase output
-    e_pos: float = 0.0
-    e_ang: float = 0.0
+    e_pos: float = None  # metres per unit of the position error head (None: s_pos)
+    e_ang: float = None  # radians per unit of the angle error head (None: s_ang); 0 flags nothing
 
     def __post_init__(self):
-        if self.e_pos == 0.0:
+        if self.e_pos is None:
             self.e_pos = self.s_pos
-        if self.e_ang == 0.0:
+        if self.e_ang is None:
             self.e_ang = self.s_ang
 
 
@@ -326,9 +332,10 @@ def normalise_targets(resid, norm):
 
 
 class Surrogate:
-    def __init__(self, norm, mlp=None, device="cuda", seed=0, k=16):
+    def __init__(self, norm, mlp=None, device="cuda", seed=0, k=16, tau=0.9):
         self.norm = norm
         self.k = k  # z-block size of the path estimate the network corrects
+        self.tau = tau  # quantile the error heads learn
         self.device = device
         self.mlp = mlp or MLP(device=device, seed=seed)
         self.lib = _lib.load()
@@ -365,8 +372,8 @@ class Surrogate:
                 tb = target.index_select(0, idx)
                 self.mlp.forward(xb, out=Y, keep=True)
                 self.lib.tt_surrogate_loss(_p(Y), NOUT, _p(tb), N_REG, _p(dY), NOUT, batch, N_REG, _p(self.w_reg),
-                                           N_ERR, _p(self.err_lo), _p(self.err_hi), NOUT, _p(loss),
-                                           _lib.stream_ptr())
+                                           N_ERR, _p(self.err_lo), _p(self.err_hi), float(self.tau), NOUT,
+                                           _p(loss), _lib.stream_ptr())
                 self.mlp.backward(dY)
                 lr_t = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * st / max(1, steps)))
                 self.mlp.adam(lr_t, wd=wd)
@@ -374,8 +381,43 @@ class Surrogate:
             l = (loss / (N // batch * batch)).cpu().numpy()
             hist.append(l)
             if log:
-                log(f"epoch {ep + 1}/{epochs}: mse {l[0]:.4g}  err-head L1 {l[1]:.4g}")
+                log(f"epoch {ep + 1}/{epochs}: mse {l[0]:.4g}  err-head pinball {l[1]:.4g}")
         return hist
+
+    def calibrate(self, feat, resid, bar_pos=1e-6, bar_ang=1e-6, margin=0.8, max_scale=1e6):
+        """Scale the error heads on held-out rays (features ``feat`` [n, 48]
+        and true residuals ``resid`` [5, n] of their path estimates, from
+        the full tracer) so that each held-out ray whose corrected position
+        or angle error exceeds ``margin`` times the bar gets a scaled bound
+        above the bar (and so is flagged by apply / trace at thresholds
+        bar_pos, bar_ang).  With no such ray the heads flag nothing.  A head
+        output of zero on such a ray caps the scale at max_scale times the
+        bar over the smallest nonzero head output, which flags nearly every
+        ray.  Returns a dict with the chosen scales and, on the held-out rays,
+        the flagged fraction and the largest unflagged errors."""
+        Y = self.predict(feat).double()
+        n = Y.shape[0]
+        r = resid.double()
+        ok = torch.isfinite(r).all(0)
+        ep = torch.maximum((r[0] - Y[:, 0] * self.norm.s_pos).abs(), (r[1] - Y[:, 1] * self.norm.s_pos).abs())
+        ea = torch.maximum((r[2] - Y[:, 2] * self.norm.s_ang).abs(), (r[3] - Y[:, 3] * self.norm.s_ang).abs())
+        out = {}
+        for name, err, head, bar in (("pos", ep, Y[:, 5].abs(), bar_pos), ("ang", ea, Y[:, 6].abs(), bar_ang)):
+            bad = ok & (err > margin * bar)
+            if bool(bad.any()):
+                hmin = head[bad].min()
+                floor = head[head > 0].min() if bool((head > 0).any()) else torch.tensor(1.0, device=head.device)
+                scale = float(bar / torch.clamp(hmin, min=float(floor) / max_scale)) * (1 + 1e-6)
+            else:
+                scale = 0.0
+            setattr(self.norm, "e_" + name, scale)
+            flagged = head * scale > bar
+            out[name] = {"scale": scale, "n_over_margin": int(bad.sum()), "flagged_frac": float(flagged.double().mean()),
+                         "max_unflagged": float(err[ok & ~flagged].max()) if bool((ok & ~flagged).any()) else 0.0}
+        both = (Y[:, 5].abs() * self.norm.e_pos > bar_pos) | (Y[:, 6].abs() * self.norm.e_ang > bar_ang) | ~ok
+        out["flagged_frac"] = float(both.double().mean())
+        out["rays"] = n
+        return out
 
     def predict(self, feat, out=None):
         """Network output [n, 8] for fp32 features [n, 48] (fused kernel)."""
@@ -426,8 +468,10 @@ class Surrogate:
             return (torch.empty((4, 0), dtype=torch.float64, device=dev), torch.empty(0, dtype=torch.float64, device=dev),
                     torch.empty(0, dtype=torch.int32, device=dev))
         main = torch.cuda.current_stream()
-        t0 = torch.cuda.Event(enable_timing=True)
-        t0.record(main)
+        t0 = None
+        if timings is not None:
+            t0 = torch.cuda.Event(enable_timing=True)
+            t0.record(main)
         if blocks is None and self.k > 1:
             blocks = block_fields(field4, grid, self.k)
         rf = torch.empty((4, n), dtype=torch.float64, device=dev)
@@ -472,19 +516,21 @@ class Surrogate:
                     self.apply(base, Y, t_pos, t_ang, out=(rf[:, a:b], phase[a:b], flag[a:b], lst, cnt), first=a)
             main.wait_stream(sp)
             main.wait_stream(sm)
-        t1 = torch.cuda.Event(enable_timing=True)
-        t1.record(main)
-        nf = int(cnt.item())
-        if nf > 0:
-            opts = _lib.TraceOpts(h, 0.0, 0)
-            self.lib.tt_trace_rk4(_p(field4), C.byref(grid), C.byref(opts), _p(s0), s0.stride(0), nf, _p(lst),
-                                  _p(rf), n, _p(phase), None, n, None, _lib.stream_ptr())
+        if timings is not None:
+            t1 = torch.cuda.Event(enable_timing=True)
+            t1.record(main)
+        # the flagged rays, their number read by the kernel from cnt: no host
+        # synchronisation, so waves of rays and graph capture stay unblocked
+        opts = _lib.TraceOpts(h, 0.0, 0)
+        self.lib.tt_trace_rk4_counted(_p(field4), C.byref(grid), C.byref(opts), _p(s0), s0.stride(0), n, _p(lst),
+                                      _p(cnt), _p(rf), n, _p(phase), _lib.stream_ptr())
+        self.n_fallback = cnt  # device int32: rays re-traced by the last call
         if timings is not None:
             t2 = torch.cuda.Event(enable_timing=True)
             t2.record(main)
             torch.cuda.synchronize()
             timings.update(estimate_and_network=t0.elapsed_time(t1), fallback=t1.elapsed_time(t2),
-                           total=t0.elapsed_time(t2), n_fallback=nf, chunks=len(bounds))
+                           total=t0.elapsed_time(t2), n_fallback=int(cnt.item()), chunks=len(bounds))
             if ev:
                 timings.update(path=ev[0].elapsed_time(ev[1]), mlp=ev[1].elapsed_time(ev[2]),
                                apply=ev[2].elapsed_time(ev[3]))
@@ -493,14 +539,14 @@ class Surrogate:
     def save(self, path):
         np.savez(path, **self.mlp.state(), mu=self.norm.mu, isd=self.norm.isd,
                  scales=np.array([self.norm.s_pos, self.norm.s_ang, self.norm.s_ph, self.norm.e_pos,
-                                  self.norm.e_ang]), k=np.array(self.k))
+                                  self.norm.e_ang]), k=np.array(self.k), tau=np.array(self.tau))
 
     @classmethod
     def load(cls, path, device="cuda"):
         st = np.load(path)
         sc = st["scales"]
         norm = Norm(st["mu"], st["isd"], float(sc[0]), float(sc[1]), float(sc[2]), float(sc[3]), float(sc[4]))
-        s = cls(norm, device=device, k=int(st["k"]))
+        s = cls(norm, device=device, k=int(st["k"]), tau=float(st["tau"]) if "tau" in st.files else 0.9)
         s.mlp.load_state(st)
         return s
 

@@ -36,10 +36,13 @@ class WaveRun:
         ``surrogate`` (surrogate.Surrogate) each wave goes through the
         surrogate + fallback form instead of the plain tracer.  ``on_wave(w,
         n)`` is called after each wave is queued (e.g. to read rf / phase).
-        Returns the number of rays re-traced by the fallback (0 for plain)."""
+        Returns the number of rays re-traced by the fallback (0 for plain),
+        summed on the device and read once, after the last wave is queued,
+        so the waves themselves never wait on the host."""
+        import torch
         p = self.p
         p.zero_histograms()
-        fallback = 0
+        fallback = torch.zeros((), dtype=torch.int64, device=p.rf.device)
         for w, off, n in self.waves():
             p.beam(beam_seed, n=n, offset=off)
             p.sort(n)
@@ -50,11 +53,11 @@ class WaveRun:
                                                blocks=blocks)
                 p.rf[:, :n].copy_(rf)
                 p.phase[:n].copy_(ph)
-                fallback += int(flag.sum())
+                fallback += flag.sum()
             p.detect(n, accumulate=True)
             if on_wave is not None:
                 on_wave(w, n)
-        return fallback
+        return int(fallback)
 
     @property
     def ray_steps(self):
