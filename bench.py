@@ -23,7 +23,7 @@ cpu_baseline = the reference's CPU path (solve_ivp RK45 through
 RegularGridInterpolator on a multiprocessing pool) timed on this host on
 samples of the same rays through the same field, as shipped (one RK45 per
 bundle, rtol 1e-3) and at the stated tolerance (one RK45 per ray at
-rtol 1e-6), each with its error against the GPU trace of the same rays.
+rtol 5e-7), each with its error against the GPU trace of the same rays.
 surrogate = the learned-correction form at its stated scale: trained on 2e7
 rays through 64 seeded 512^3 realisations, error heads calibrated on one more,
 evaluated on 1e8 rays over 8 held-out realisations; z-block path estimate +
@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU rounds per mode")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="tracer kernel: 0 cell-cached, 1 per-stage gathers")
+    ap.add_argument("--plane-forms", action="store_true",
+                    help="build the plane-form field for the tracer's cell cache (default off at 512^3 / 1e7 rays)")
     ap.add_argument("--workload", choices=["single", "large"], default="single",
                     help="single: 512^3 / 1e7 rays (default); large: 1024^3 / 1e8 rays streamed in 1e7-ray waves")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate + fallback leg")
@@ -96,7 +98,7 @@ def main():
 
     from turbulence_tracing_amd.pipeline import Config, Pipeline
 
-    cfg = Config(n=args.n, n_rays=int(args.rays), variant=args.variant)
+    cfg = Config(n=args.n, n_rays=int(args.rays), variant=args.variant, plane_forms=args.plane_forms)
     p = Pipeline(cfg)
     log(f"pipeline ready: {cfg.n}^3, {cfg.n_rays} rays")
     stream = torch.cuda.current_stream()
@@ -195,7 +197,7 @@ def main_large(args):
     import torch
     from turbulence_tracing_amd.waves import WaveRun, large_volume_config
     steps = args.steps if args.steps != 5 else 2
-    cfg = large_volume_config(variant=args.variant)
+    cfg = large_volume_config(variant=args.variant, plane_forms=args.plane_forms)
     run = WaveRun(cfg, 100_000_000, 10_000_000)
     run.field(7)
     log(f"large volume ready: {cfg.n}^3, {run.total:.0e} rays in {run.n_waves} waves")
@@ -555,7 +557,7 @@ def cpu_baseline(base, p, cfg, bundle, budget, gpu_rays_per_s):
         "sample": f"rays of the same beam through the same {cfg.n}^3 field on {base.P} processes "
                   f"(os.cpu_count() = {base.cpu_count}, affinity {base.affinity}; at most {16} used, the host-core "
                   f"share of one GPU): value = the stated-tolerance mode ({head['rays']} rays, one solve_ivp RK45 "
-                  f"per ray at rtol 1e-6, within 1e-3 of the RMS deflection of a tight solve), in ray-steps of the "
+                  f"per ray at rtol 5e-7, within 1e-3 of the RMS deflection of a tight solve), in ray-steps of the "
                   f"GPU's problem (rays x {cfg.steps_per_ray}); the as-shipped mode ({modes['as_shipped']['rays']} "
                   f"rays, one RK45 per {bundle}-ray bundle at rtol 1e-3) is in modes",
         "modes": modes,

@@ -115,6 +115,15 @@ int tt_grad_stencil(const void *field, int32_t dtype, int64_t sx, int64_t sy, in
                     const tt_grid *grid, double alpha, double beta, double ne_max, double k0,
                     const double *sumsq, int64_t count, float *out4, void *stream);
 
+/* Plane forms of a node field for the tracer's cell cache (the trilinear
+ * interpolation of calc_dndr's gradients, PT:239-241, reorganised per z-plane
+ * cell): forms[iz][iy][ix] = 4 float4 {A, B, C, E} with A = f(ix, iy),
+ * B = f(ix+1, iy) - A, C = f(ix, iy+1) - A, E = (f(ix+1, iy+1) - f(ix, iy+1))
+ * - B on plane iz, so the field on that plane cell is A + B tx + C ty +
+ * E tx ty (B = C = E = 0 on the last column and row).  64 bytes per node;
+ * pass the result as tt_trace_opts.forms. */
+int tt_plane_forms(const float *field4, const tt_grid *grid, float *forms, void *stream);
+
 /* ------------------------------------------------------------------ tracer
  * Replaces solve + ray_at_exit + the dndr/dsdt calls inside solve_ivp
  * (particle_tracking/particle_tracker.py:243-256, 312-380, 398-419).
@@ -127,11 +136,18 @@ int tt_grad_stencil(const void *field, int32_t dtype, int64_t sx, int64_t sy, in
  * exit plane (the field is zero outside the grid, PT:239-241).
  * status (nullable, int32): 0 ok, 1 ray not moving forward in z, 2 NaN.
  * variant 0: each lane caches its current cell as a trilinear polynomial in
- * registers (the fast path); variant 1: eight float4 gathers per RK4 stage. */
+ * registers (the fast path); variant 1: eight float4 gathers per RK4 stage.
+ * With h = 1/2 and variant 0 the phase is summed by the composite Simpson
+ * rule over the lattice points (the ray's states on the z-planes and
+ * half-planes), elsewhere by the RK4 weights.  forms (nullable, device,
+ * 16-byte aligned): the plane forms tt_plane_forms writes for field4; with
+ * h = 1/2, variant 0 and no sf, the cell cache then loads them instead of
+ * forming them from field4 (same results, bit for bit, fewer instructions). */
 typedef struct {
     double h;
     double cT;
     int32_t variant;
+    const float *forms;
 } tt_trace_opts;
 int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *opts,
                  const double *s0, int64_t ld_s0, int64_t n, const int32_t *order,

@@ -12,9 +12,9 @@ reference's dsdt (PT:398-419), then ray_at_exit (PT:333-380).  Two modes:
   * "as_shipped": one solve_ivp RK45 over the whole bundle at scipy's default
     tolerances (PT:312-331, rtol 1e-3 on one error norm over all rays);
   * "stated_tolerance": one solve_ivp RK45 per ray at rtol = RTOL_STATED,
-    the loosest per-ray tolerance whose exit angles stay within 1e-3 of the
-    RMS deflection of a tight (rtol 1e-11) solve at 512^3 (rtol 1e-5 leaves
-    8.7e-3 of it, 1e-6 leaves 9.9e-4).
+    a per-ray tolerance whose exit angles stay within 1e-3 of the RMS
+    deflection of a tight (rtol 1e-11) solve at 512^3: rtol 1e-5 leaves
+    8.7e-3 of it and 1e-6 leaves 1.0-1.3e-3 (on two samples), so 5e-7.
 
 The gradient arrays live in POSIX shared memory, written by the parent, so
 workers (forked before the parent touches the GPU) attach by name.
@@ -25,7 +25,7 @@ from multiprocessing import get_context, shared_memory
 
 import numpy as np
 
-RTOL_STATED = 1e-6
+RTOL_STATED = 5e-7
 MAX_PROCESSES = 16  # the host-core share of one GPU on the measuring machine
 
 _G = {}

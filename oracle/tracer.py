@@ -275,6 +275,17 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
     outside, PT:239-241); a ray starting between lattice points takes one
     partial first step.
 
+    The phase is summed with the RK4 weights of each step, except for
+    h = 1/2, where it follows the GPU's cell-cached march: a cell [c, c + 1]
+    whose two steps each keep all four stage points inside the x / y column
+    cell they start in takes Simpson's rule on the ray's points,
+    (f(c) + 4 f(c + 1/2) + f(c + 1)) / 6; a step whose stage points cross a
+    face takes the RK4 weights, and the other step of such a cell Simpson's
+    rule on its half cell with the midpoint integrand taken a quarter cell
+    back along the ray from that step's end (or, for the cell's first step,
+    from its second step's start).  A partial first step and a leading odd
+    step keep the RK4 weights.
+
     Returns rf (4 x N: x, theta, y, phi_angle), phase (N), sf (6 x N state at
     c t = cT, straight continuation past the exit plane), status (N; 0 ok,
     1 non-forward ray, 2 NaN input)."""
@@ -330,7 +341,7 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
         yn = [a + hh / 6.0 * (b + 2 * c + 2 * d + e) for a, b, c, d, e in zip(y, k1, k2, k3, k4)]
         dp = hh / 6.0 * (p1 + 2 * p2 + 2 * p3 + p4)
         yn = [np.where(act, a, b) for a, b in zip(yn, y)]
-        return yn, np.where(act, dp, 0.0)
+        return yn, np.where(act, dp, 0.0), np.where(act, p1, 0.0)
 
     with np.errstate(divide="ignore", invalid="ignore"):
         # lattice start index per ray; a partial first step if off-lattice
@@ -339,14 +350,64 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
         part = ok & (kstart * h - Zs > 1e-12) & (Zs < Zend)
         if part.any():
             hh = np.where(part, kstart * h - Zs, 0.0)
-            y, dp = rk4(Zs, y, hh, part)
+            y, dp, _ = rk4(Zs, y, hh, part)
             phase += dp
+        cellwise = m == 2
+        kfirst = kstart + (kstart % 2) if cellwise else np.full(N, K + 1)
+        pend = np.zeros(N)   # weight owed to the next step's start-point integrand
+        phi0 = np.zeros(N)   # the cell's first step's start-point integrand
+        s1 = np.zeros(N, dtype=bool)
+
+        def column_cell(yy):
+            return np.floor(yy[0]), np.floor(yy[1])
+
+        def phi_back(Z, yy, back):
+            """integrand at plane Z, a quarter cell back along the ray from yy"""
+            px, py, U, V, W, tau = yy
+            return rhs(Z, [px - back * sx * U / W, py - back * sy * V / W, U, V, W, tau])[1]
+
         for k in range(K):
             act = ok & (kstart <= k)
             if not act.any():
                 continue
-            y, dp = rk4(k * h, y, h, act)
-            phase += dp
+            ys0 = y
+            y, dp, p1 = rk4(k * h, ys0, h, act)
+            if not cellwise:
+                phase += dp
+                continue
+            # stage points of this step, for the smooth (no face crossed) test
+            k1, _ = rhs(k * h, ys0)
+            y2 = [a_ + 0.5 * h * b_ for a_, b_ in zip(ys0, k1)]
+            k2, _ = rhs(k * h + 0.5 * h, y2)
+            y3 = [a_ + 0.5 * h * b_ for a_, b_ in zip(ys0, k2)]
+            k3, _ = rhs(k * h + 0.5 * h, y3)
+            y4 = [a_ + h * b_ for a_, b_ in zip(ys0, k3)]
+            cx, cy = column_cell(ys0)
+            smooth = np.ones(N, dtype=bool)
+            for yy in (y2, y3, y4):
+                ex, ey = column_cell(yy)
+                smooth &= (ex == cx) & (ey == cy)
+            lead = act & (k < kfirst)          # a leading odd step: RK4
+            phase += np.where(lead, dp, 0.0)
+            cw = act & (k >= kfirst)
+            if k % 2 == 0:
+                add = pend * p1 + np.where(smooth, 0.0, dp)
+                phase += np.where(cw, add, 0.0)
+                pend = np.where(cw, 0.0, pend)
+                s1 = np.where(cw, smooth, s1)
+                phi0 = np.where(cw, p1, phi0)
+            else:
+                Zh = k * h
+                pa = phi_back(Zh - 0.25, ys0, 0.25)   # the first step's midpoint
+                pb = phi_back(Zh + 0.25, y, 0.25)    # this step's midpoint
+                add = np.where(s1 & smooth, (phi0 + 4 * p1) / 6.0,
+                               np.where(s1, (phi0 + 4 * pa + p1) / 12.0 + dp,
+                                        np.where(smooth, (p1 + 4 * pb) / 12.0, dp)))
+                phase += np.where(cw, add, 0.0)
+                pend = np.where(cw, np.where(s1 & smooth, 1.0 / 6.0, np.where(~s1 & smooth, 1.0 / 12.0, 0.0)), pend)
+        if cellwise:  # the exit point's integrand, owed by the last step
+            _, p_end = rhs(Zend, y)
+            phase += np.where(ok & (kfirst < K), pend * p_end, 0.0)
 
     px, py, U, V, W, tau = y
     xe = x0 + px * dx

@@ -201,6 +201,55 @@ def test_trace_unsorted_equals_sorted(lib):
     np.testing.assert_array_equal(a[2], b[2])
 
 
+def test_plane_forms_match_numpy(lib):
+    """tt_plane_forms writes, per node, the bilinear form of its z-plane
+    cell, A = f00, B = f10 - f00, C = f01 - f00, E = (f11 - f01) - B, with
+    the same float32 subtractions as numpy (bit for bit), and zero B, C, E on
+    the last column and row."""
+    from turbulence_tracing_amd import _lib
+    nx, ny, nz = 13, 9, 5
+    rs = np.random.RandomState(3)
+    f4 = rs.randn(nz, ny, nx, 4).astype(np.float32)
+    grid = _lib.Grid(nx, ny, nz, 0, 0, 0, 1, 1, 1)
+    dev = torch.as_tensor(f4, device=DEV)
+    out = torch.full((nz, ny, nx, 16), 7.0, dtype=torch.float32, device=DEV)
+    lib.tt_plane_forms(dev.data_ptr(), __import__("ctypes").byref(grid), out.data_ptr(), _lib.stream_ptr())
+    got = out.cpu().numpy().reshape(nz, ny, nx, 4, 4)
+    A = f4
+    B = np.zeros_like(f4)
+    Cc = np.zeros_like(f4)
+    E = np.zeros_like(f4)
+    B[:, :-1, :-1] = f4[:, :-1, 1:] - f4[:, :-1, :-1]
+    Cc[:, :-1, :-1] = f4[:, 1:, :-1] - f4[:, :-1, :-1]
+    E[:, :-1, :-1] = (f4[:, 1:, 1:] - f4[:, 1:, :-1]) - B[:, :-1, :-1]
+    for i, want in enumerate((A, B, Cc, E)):
+        np.testing.assert_array_equal(got[..., i, :], want)
+
+
+@pytest.mark.parametrize("n", [64, 97])
+def test_trace_with_plane_forms_is_bit_identical(lib, n):
+    """The cell cache loading the plane forms (tt_plane_forms) instead of
+    forming them from the node field: the same exit rays and phase, bit for
+    bit, through the Pipeline."""
+    from turbulence_tracing_amd.pipeline import Config, Pipeline
+    res = []
+    for forms in (True, False):
+        p = Pipeline(Config(n=n, n_rays=30000, pix=64, plane_forms=forms))
+        p.run(seed=4)
+        torch.cuda.synchronize()
+        res.append((p.rf.cpu().numpy(), p.phase.cpu().numpy(), p.order.cpu().numpy()))
+    (rf_a, ph_a, o_a), (rf_b, ph_b, o_b) = res
+    # per ray in beam order (the order within a column is not fixed)
+    A, Bv = np.empty_like(rf_a), np.empty_like(rf_b)
+    A[:, o_a] = rf_a
+    Bv[:, o_b] = rf_b
+    np.testing.assert_array_equal(A, Bv)
+    pa, pb = np.empty_like(ph_a), np.empty_like(ph_b)
+    pa[o_a] = ph_a
+    pb[o_b] = ph_b
+    np.testing.assert_array_equal(pa, pb)
+
+
 def test_trace_golden_cube_vs_reference_solve(lib, golden):
     """The reference's own as-shipped trace (golden) sits within its RK45
     tolerance of the GPU trace of the same rays."""

@@ -34,7 +34,7 @@ class Spectrum(C.Structure):
 
 
 class TraceOpts(C.Structure):
-    _fields_ = [("h", C.c_double), ("cT", C.c_double), ("variant", C.c_int32)]
+    _fields_ = [("h", C.c_double), ("cT", C.c_double), ("variant", C.c_int32), ("forms", C.c_void_p)]
 
 
 class Optic(C.Structure):
@@ -91,6 +91,7 @@ _SIGNATURES = {
     "tt_field_from_noise": [P, P, P, P, P, P, P, P],
     "tt_power_spectrum": [P, P, I32, I32, D, D, P, P, P, P],
     "tt_grad_stencil": [P, I32, I64, I64, I64, C.POINTER(Grid), D, D, D, D, P, I64, P, P],
+    "tt_plane_forms": [P, C.POINTER(Grid), P, P],
     "tt_trace_rk4": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, I64, P, P, I64, P, P],
     "tt_trace_rk4_counted": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, P, I64, P, P],
     "tt_sample_field": [P, C.POINTER(Grid), P, I64, I64, P, I64, P],

@@ -9,6 +9,8 @@
 //   Phi = k0 dz (sqrt(1 - ne/nc) - 1)  (phase per z-cell)
 // Algorithmic traffic: sizeof(T) read + 16 B written per node (neighbour
 // re-reads hit L1/L2).
+#include <algorithm>
+
 #include "tt_internal.h"
 
 namespace {
@@ -85,6 +87,51 @@ __global__ __launch_bounds__(256) void grad_kernel(GradArgs<T> a) {
     }
 }
 
+// Plane forms of the node field for the tracer's cell cache: node (ix, iy,
+// iz) gets the bilinear form of the z-plane cell whose lower corner it is,
+//   A = f00, B = f10 - f00, C = f01 - f00, E = (f11 - f01) - B,
+// so that f(tx, ty) = A + B tx + C ty + E tx ty on that plane (the last
+// column and row, which start no cell, get A = f and B = C = E = 0).  The
+// subtractions are those the tracer would do per cell, so the forms equal
+// the ones it forms itself, bit for bit.  One thread per output float4
+// (component j = t % 4 of node t / 4), so a wave's stores are one contiguous
+// 1 KiB run; the neighbour loads hit L1 / L2.  Algorithmic traffic: 16 B read
+// + 64 B written per node.
+__device__ __forceinline__ void plane_form(const float4 *__restrict__ f, float4 *__restrict__ out, int nx, int ny,
+                                           int64_t t) {
+    const int j = (int)(t & 3);
+    const int64_t i = t >> 2;
+    const int ix = (int)(i % nx);
+    const int iy = (int)((i / nx) % ny);
+    const float4 a = f[i];
+    float4 o = a;
+    if (j > 0) {
+        o = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (ix < nx - 1 && iy < ny - 1) {
+            const float4 f10 = f[i + 1], f01 = f[i + nx];
+            const float4 b = make_float4(f10.x - a.x, f10.y - a.y, f10.z - a.z, f10.w - a.w);
+            if (j == 1) {
+                o = b;
+            } else if (j == 2) {
+                o = make_float4(f01.x - a.x, f01.y - a.y, f01.z - a.z, f01.w - a.w);
+            } else {
+                const float4 f11 = f[i + nx + 1];
+                o = make_float4((f11.x - f01.x) - b.x, (f11.y - f01.y) - b.y, (f11.z - f01.z) - b.z,
+                                (f11.w - f01.w) - b.w);
+            }
+        }
+    }
+    // streamed to HBM (nontemporal): written once, read by the tracer later
+    __builtin_nontemporal_store((f32x4){o.x, o.y, o.z, o.w}, reinterpret_cast<f32x4 *>(out + t));
+}
+
+__global__ __launch_bounds__(256) void plane_forms_kernel(const float4 *__restrict__ f, float4 *__restrict__ out,
+                                                          int nx, int ny, int64_t n4) {
+    // grid-stride: a launch holds fewer than 2^32 work-items
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += (int64_t)gridDim.x * blockDim.x)
+        plane_form(f, out, nx, ny, t);
+}
+
 template <typename T>
 int launch(const void *field, int64_t sx, int64_t sy, int64_t sz, const tt_grid *g, double alpha, double beta,
            double ne_max, double k0, const double *sumsq, int64_t count, float *out4, hipStream_t st) {
@@ -130,4 +177,17 @@ extern "C" int tt_grad_stencil(const void *field, int32_t dtype, int64_t sx, int
     if (dtype == 1) return launch<double>(field, sx, sy, sz, grid, alpha, beta, ne_max, k0, sumsq, count, out4, st);
     tt::set_error("tt_grad_stencil: dtype must be 0 (float32) or 1 (float64), got %d", dtype);
     return TT_ERR_ARG;
+}
+
+extern "C" int tt_plane_forms(const float *field4, const tt_grid *grid, float *forms, void *stream) {
+    TT_REQUIRE(field4 && grid && forms, "tt_plane_forms: null pointer");
+    TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 1, "tt_plane_forms: grid needs >= 2 nodes in x and y");
+    TT_REQUIRE(forms != field4, "tt_plane_forms: forms must not alias field4");
+    const int64_t n4 = 4 * (int64_t)grid->nx * grid->ny * grid->nz;
+    const int64_t blocks = std::min<int64_t>(tt::ceil_div(n4, 256), (int64_t)1 << 22);
+    hipLaunchKernelGGL(plane_forms_kernel, dim3((unsigned)blocks), dim3(256), 0,
+                       tt::as_stream(stream), reinterpret_cast<const float4 *>(field4),
+                       reinterpret_cast<float4 *>(forms), grid->nx, grid->ny, n4);
+    TT_LAUNCH_CHECK("tt_plane_forms launch");
+    return TT_OK;
 }

@@ -30,6 +30,7 @@ namespace {
 
 struct TraceArgs {
     const float4 *field;
+    const float4 *forms;  // optional plane forms (tt_plane_forms): 4 float4 per node
     int nx, ny, nz;
     float fx1, fy1, fz1;  // n - 1 as float (upper bounds, inclusive)
     double x0, y0, z0, dx, dy, dz;
@@ -126,6 +127,42 @@ __device__ __forceinline__ float rk4_step(const TraceArgs &a, float pxb, float p
     y.Wd += h6 * (k1.Wd + 2.f * (k2.Wd + k3.Wd) + k4.Wd);
     y.tr += h6 * (k1.tr + 2.f * (k2.tr + k3.tr) + k4.tr);
     return h6 * (p1 + 2.f * (p2 + p3) + p4) * vf;
+}
+
+// One classic RK4 step as rk4_step, returning the phase integrand
+// Phi |v| / W at the step's start point and, in dph, the step's phase by the
+// RK4 weights (the h = 1/2 march uses one or the other).
+__device__ __forceinline__ float rk4_step_phi(const TraceArgs &a, float pxb, float pyb, float Z, float hh, State &y,
+                                              float &dph) {
+    State k1, k2, k3, k4;
+    float p1, p2, p3, p4, vf1 = 0.f, vf = 0.f, unused;
+    const float h2 = 0.5f * hh;
+    rhs<true>(a, pxb, pyb, Z, y, k1, p1, vf1);
+    rhs<false>(a, pxb, pyb, Z + h2, axpy(y, h2, k1), k2, p2, unused);
+    rhs<true>(a, pxb, pyb, Z + h2, axpy(y, h2, k2), k3, p3, vf);
+    rhs<false>(a, pxb, pyb, Z + hh, axpy(y, hh, k3), k4, p4, unused);
+    const float h6 = hh * (1.0f / 6.0f);
+    y.qx += h6 * (k1.qx + 2.f * (k2.qx + k3.qx) + k4.qx);
+    y.qy += h6 * (k1.qy + 2.f * (k2.qy + k3.qy) + k4.qy);
+    y.U += h6 * (k1.U + 2.f * (k2.U + k3.U) + k4.U);
+    y.V += h6 * (k1.V + 2.f * (k2.V + k3.V) + k4.V);
+    y.Wd += h6 * (k1.Wd + 2.f * (k2.Wd + k3.Wd) + k4.Wd);
+    y.tr += h6 * (k1.tr + 2.f * (k2.tr + k3.tr) + k4.tr);
+    dph = h6 * (p1 + 2.f * (p2 + p3) + p4) * vf;
+    return p1 * vf1;
+}
+
+// The phase integrand Phi |v| / W by trilinear gathers at plane Z (cells),
+// at the position of state s moved back by `back` cells of z along its
+// direction (used at the ends of the march and, in the rare cells where one
+// step crossed a face, at the other step's midpoint).
+__device__ __forceinline__ float phi_at(const TraceArgs &a, float pxb, float pyb, float Z, const State &s,
+                                        float back = 0.f) {
+    const float W = 1.0f + s.Wd;
+    const float iW = 1.0f / W;
+    const float qx = fmaf(-back * s.U, iW, s.qx), qy = fmaf(-back * s.V, iW, s.qy);
+    const float4 g = sample(a, fmaf(a.sx, qx, pxb), fmaf(a.sy, qy, pyb), Z);
+    return g.w * path_factor(s, W, iW);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,28 +292,29 @@ __device__ __forceinline__ State axpy_r(const State &y, float c, const State &s,
                  fmaf(c, g.y, y.V),  fmaf(c, g.z, y.Wd), TAU ? fmaf(c, -s.Wd, y.tr) : y.tr};
 }
 
-// one RK4 step of h = 1/2 over the planes z0 (start), zm (middle), z1 (end)
+// One RK4 step of h = 1/2 over the planes z0 (start), zm (middle), z1 (end).
+// Returns the phase integrand Phi |v| / W at the step's start point: the
+// phase is summed by the composite Simpson rule over the march's lattice
+// points (march_m2), so only the first stage needs the Phi channel and the
+// others' Phi (and the Phi channel of the zm and z1 planes) is never formed.
 template <bool TAU>
 __device__ __forceinline__ float rk4_half_z(const TraceArgs &a, const ZP &z0, const ZP &zm, const ZP &z1, float ox,
                                             float oy, State &y, float &mn, float &mx) {
     const State zero{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float4 g;
-    float iW, pacc, vf = 0.f;
-    rhs_z<false>(a, z0, ox, oy, y, g, iW, vf, mn, mx);
+    float iW, vf = 0.f, unused;
+    rhs_z<true>(a, z0, ox, oy, y, g, iW, vf, mn, mx);
+    const float phi = g.w * vf;
     State acc = axpy_r<TAU>(zero, iW, y, g);  // k1 + 2 k2 + 2 k3 + k4, k = r / W
-    pacc = g.w;
     const State s2 = axpy_r<TAU>(y, 0.25f * iW, y, g);
-    rhs_z<false>(a, zm, ox, oy, s2, g, iW, vf, mn, mx);
+    rhs_z<false>(a, zm, ox, oy, s2, g, iW, unused, mn, mx);
     acc = axpy_r<TAU>(acc, 2.f * iW, s2, g);
-    pacc = fmaf(2.f, g.w, pacc);
     const State s3 = axpy_r<TAU>(y, 0.25f * iW, s2, g);
-    rhs_z<true>(a, zm, ox, oy, s3, g, iW, vf, mn, mx);
+    rhs_z<false>(a, zm, ox, oy, s3, g, iW, unused, mn, mx);
     acc = axpy_r<TAU>(acc, 2.f * iW, s3, g);
-    pacc = fmaf(2.f, g.w, pacc);
     const State s4 = axpy_r<TAU>(y, 0.5f * iW, s3, g);
-    rhs_z<false>(a, z1, ox, oy, s4, g, iW, vf, mn, mx);
+    rhs_z<false>(a, z1, ox, oy, s4, g, iW, unused, mn, mx);
     acc = axpy_r<TAU>(acc, iW, s4, g);
-    pacc += g.w;
     const float h6 = 0.5f * (1.0f / 6.0f);
     y.qx = fmaf(h6, acc.qx, y.qx);
     y.qy = fmaf(h6, acc.qy, y.qy);
@@ -284,13 +322,24 @@ __device__ __forceinline__ float rk4_half_z(const TraceArgs &a, const ZP &z0, co
     y.V = fmaf(h6, acc.V, y.V);
     y.Wd = fmaf(h6, acc.Wd, y.Wd);
     if (TAU) y.tr = fmaf(h6, acc.tr, y.tr);
-    return h6 * pacc * vf;
+    return phi;
 }
 
 // The cell as plane forms: z0 on its lower z-plane and D = (upper - lower),
 // so the plane at tz is z0 + tz D (same coefficients as load_cell: A, B, Cc,
-// E = a, b, c, e and D = d, f, g, h).
+// E = a, b, c, e and D = d, f, g, h).  With FORMS the two planes' forms are
+// loaded as written by tt_plane_forms (the same subtractions, so the same
+// values bit for bit) and only D is formed here.
+template <bool FORMS>
 __device__ __forceinline__ void load_planes(const TraceArgs &A, int ix, int iy, int iz, ZP &z0, ZP &D) {
+    if (FORMS) {
+        const int64_t plane4 = 4 * (int64_t)A.nx * A.ny;
+        const float4 *p = A.forms + 4 * (((int64_t)iz * A.ny + iy) * A.nx + ix);
+        z0 = ZP{p[0], p[1], p[2], p[3]};
+        const float4 a1 = p[plane4], b1 = p[plane4 + 1], c1 = p[plane4 + 2], e1 = p[plane4 + 3];
+        D = ZP{f4sub(a1, z0.A), f4sub(b1, z0.B), f4sub(c1, z0.Cc), f4sub(e1, z0.E)};
+        return;
+    }
     const int64_t plane = (int64_t)A.nx * A.ny;
     const float4 *p = A.field + ((int64_t)iz * A.ny + iy) * A.nx + ix;
     const float4 c000 = p[0], c100 = p[1], c010 = p[A.nx], c110 = p[A.nx + 1];
@@ -310,6 +359,58 @@ __device__ __forceinline__ ZP zstep(const ZP &z, float dt, const ZP &D) {
     return ZP{f4fma(D.A, dt, z.A), f4fma(D.B, dt, z.B), f4fma(D.Cc, dt, z.Cc), f4fma(D.E, dt, z.E)};
 }
 
+// Phase bookkeeping of the h = 1/2 march, per lane.  The phase
+// int Phi |v| / W dZ is summed cell by cell: a cell [c, c + 1] whose two
+// steps both ran from the cell cache (no stage point left the cell, so the
+// integrand is the smooth cubic the trilinear field makes along the ray) takes
+// Simpson's rule on the ray's points, (f(c) + 4 f(c + 1/2) + f(c + 1)) / 6; a
+// step that crossed a face (gathered) takes the RK4 weights; the other step
+// of such a cell takes Simpson's rule on its half cell, with its midpoint
+// integrand gathered at the midpoint one quarter cell back along the ray.  A
+// step needs the Phi channel at its first stage only; f(c + 1) is owed to the
+// next step's start point (pend).
+struct PhaseLedger {
+    float pend = 0.f;  // weight owed to the next step's start-point integrand
+    float phi0 = 0.f;  // the cell's first step's start-point integrand
+    bool s1 = false;   // the cell's first step ran from the cache
+};
+
+// the cell's first step (even k): start-point integrand phi, smooth = ran
+// from the cache, dph = its RK4 phase (used when it did not)
+__device__ __forceinline__ void ledger_first(PhaseLedger &L, double &ph, float phi, bool smooth, float dph) {
+    float add = L.pend * phi;
+    if (!smooth) add += dph;
+    L.pend = 0.f;
+    L.s1 = smooth;
+    L.phi0 = phi;
+    ph += (double)add;
+}
+
+// the cell's second step (odd k) from state ys0 (its start) to ys1 (its end)
+__device__ __forceinline__ void ledger_second(const TraceArgs &a, float pxb, float pyb, int k, PhaseLedger &L,
+                                              double &ph, float phi, bool smooth, float dph, const State &ys0,
+                                              const State &ys1) {
+    constexpr float k6 = 1.0f / 6.0f, k12 = 1.0f / 12.0f;
+    const float Zh = (float)k * 0.5f;  // the cell's half plane
+    float add;
+    if (L.s1 && smooth) {
+        add = k6 * fmaf(4.0f, phi, L.phi0);
+        L.pend = k6;
+    } else if (L.s1) {  // the first half took the cache, this one crossed a face
+        const float pm = phi_at(a, pxb, pyb, Zh - 0.25f, ys0, 0.25f);
+        add = k12 * (L.phi0 + fmaf(4.0f, pm, phi)) + dph;
+        L.pend = 0.f;
+    } else if (smooth) {  // the first half crossed a face, this one took the cache
+        const float pm = phi_at(a, pxb, pyb, Zh + 0.25f, ys1, 0.25f);
+        add = k12 * fmaf(4.0f, pm, phi);
+        L.pend = k12;
+    } else {
+        add = dph;
+        L.pend = 0.f;
+    }
+    ph += (double)add;
+}
+
 // Steps k .. K-1 for h = 1/2 (two steps per cell).  A cell is held as its
 // lower plane form and the upper-minus-lower difference (32 registers); the
 // stage planes are z0 + D/4, z0 + D/2, then stepped by D/4 from there.  The
@@ -319,10 +420,11 @@ __device__ __forceinline__ ZP zstep(const ZP &z, float dt, const ZP &D) {
 // gathers.  If the redone step was the cell's first, the next iteration runs
 // the second step from the cache: the lanes that redid it locate the cell
 // they are now in, the others reload their own cell (cell and planes are
-// formed the same way on both routes, so what a lane computes never depends
-// on the other lanes of its wave).  A start between z-planes (odd k) takes
-// one gathered step first.
-template <bool TAU>
+// formed the same way on both routes, and the phase ledger follows each
+// lane's own steps, so what a lane computes never depends on the other lanes
+// of its wave).  A start between z-planes (odd k) takes one gathered step
+// first (its phase by the RK4 weights).
+template <bool TAU, bool FORMS>
 __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
                                          double &ph) {
     bool good = true;
@@ -331,6 +433,8 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
         good = ys.Wd > -1.0f;
         ++k;
     }
+    if (k >= K) return good;
+    PhaseLedger L;
     float ox = 0.f, oy = 0.f;  // entry point relative to the current cell
     bool inside = false;       // the cell lies in the grid (x, y)
     bool moved = false;        // this lane's last step was redone because it left its cell
@@ -345,7 +449,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
     };
     while (k < K) {
         State yn;
-        float dph;
+        float phi;
         bool ok;
         if (k & 1) {  // the second step of a cell whose first step was redone
             int ix, iy;
@@ -356,15 +460,16 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
                 iy = (int)rintf(pyb - oy);
             }
             ZP z0, D;
-            load_planes(a, ix, iy, k >> 1, z0, D);
+            load_planes<FORMS>(a, ix, iy, k >> 1, z0, D);
             float mn = inside ? 0.f : -1.f, mx = 0.f;
             const ZP zh = zstep(z0, 0.5f, D);
             const ZP z3 = zstep(zh, 0.25f, D);
             const ZP z1 = zstep(z3, 0.25f, D);
             yn = ys;
-            dph = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
+            phi = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
             ok = mn >= 0.f && mx <= 1.f;
         } else {  // whole cells from the cache
+            bool done = false;
             for (;;) {
                 int ix, iy;
                 // raised wave priority while the cell's loads are issued, so
@@ -373,43 +478,52 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
                 __builtin_amdgcn_s_setprio(1);
                 locate(ix, iy);
                 ZP z0, D;
-                load_planes(a, ix, iy, k >> 1, z0, D);
+                load_planes<FORMS>(a, ix, iy, k >> 1, z0, D);
                 __builtin_amdgcn_s_setprio(0);
                 float mn = inside ? 0.f : -1.f, mx = 0.f;
                 const ZP zq = zstep(z0, 0.25f, D);
                 const ZP zh = zstep(z0, 0.5f, D);
                 yn = ys;
-                dph = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx);
+                phi = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx);
                 ok = mn >= 0.f && mx <= 1.f;
                 if (__ballot(!ok) != 0) break;  // redo the cell's first step below
                 ys = yn;
-                ph += (double)dph;
+                ledger_first(L, ph, phi, true, 0.f);
                 ++k;
                 good = good && (ys.Wd > -1.0f);
                 const ZP z3 = zstep(zh, 0.25f, D);
                 const ZP z1 = zstep(z3, 0.25f, D);
                 yn = ys;
-                dph = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
+                phi = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
                 ok = mn >= 0.f && mx <= 1.f;
                 if (__ballot(!ok) != 0) break;  // redo the cell's second step below
+                ledger_second(a, pxb, pyb, k, L, ph, phi, true, 0.f, ys, yn);
                 ys = yn;
-                ph += (double)dph;
                 ++k;
                 good = good && (ys.Wd > -1.0f);
-                if (k >= K) return good;
+                if (k >= K) {
+                    done = true;
+                    break;
+                }
             }
+            if (done) break;
         }
         // step k: the lanes that left their cell redo it with gathers
+        float dph = 0.f;
         if (!ok) {
             yn = ys;
-            dph = rk4_step(a, pxb, pyb, (float)k * 0.5f, 0.5f, yn);
+            phi = rk4_step_phi(a, pxb, pyb, (float)k * 0.5f, 0.5f, yn, dph);
         }
+        if (k & 1)
+            ledger_second(a, pxb, pyb, k, L, ph, phi, ok, dph, ys, yn);
+        else
+            ledger_first(L, ph, phi, ok, dph);
         moved = !ok;
         ys = yn;
-        ph += (double)dph;
         ++k;
         good = good && (ys.Wd > -1.0f);
     }
+    if (L.pend != 0.f) ph += (double)(L.pend * phi_at(a, pxb, pyb, (float)K * 0.5f, ys));  // exit point
     return good;
 }
 
@@ -467,8 +581,9 @@ __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, floa
 }
 
 // One ray: the t-th of the launch (a.order[t] when given).  CACHED selects
-// the cell-cached march (M2: its h = 1/2 form, which requires a.m == 2).
-template <bool CACHED, bool M2, bool TAU>
+// the cell-cached march (M2: its h = 1/2 form, which requires a.m == 2;
+// FORMS: its planes from the plane-form field a.forms).
+template <bool CACHED, bool M2, bool TAU, bool FORMS = false>
 __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
     const int64_t r = a.order ? (int64_t)a.order[t] : t;
     const double *s = a.s0 + r;
@@ -510,7 +625,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
             if (CACHED) {
                 // partial first step (off-lattice start) with gathers, then the lattice
                 if (h0 > 1e-12) ph += (double)rk4_step(a, pxb, pyb, (float)Zs, (float)h0, ys);
-                if (!(M2 ? march_m2<TAU>(a, pxb, pyb, kstart, K, ys, ph)
+                if (!(M2 ? march_m2<TAU, FORMS>(a, pxb, pyb, kstart, K, ys, ph)
                          : march_cached<TAU>(a, pxb, pyb, kstart, K, ys, ph)))
                     st = 1;
             } else {
@@ -554,13 +669,13 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
 
 // The h = 1/2 form fits 96 VGPRs (five waves per SIMD) with one 8-byte spill
 // outside the step loop.
-template <bool CACHED, bool M2, bool TAU>
+template <bool CACHED, bool M2, bool TAU, bool FORMS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M2 ? 5 : 1))) void trace_rk4_kernel(TraceArgs a) {
     const int b = tt::xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int64_t t = (int64_t)b * blockDim.x + threadIdx.x;
     if (t >= a.n) return;
     if (a.n_dev && t >= (int64_t)*a.n_dev) return;
-    trace_ray<CACHED, M2, TAU>(a, t);
+    trace_ray<CACHED, M2, TAU, FORMS>(a, t);
 }
 
 // ------------------------------------------------------------- ensembles
@@ -684,6 +799,8 @@ int trace_launch(const float *field4, const tt_grid *grid, const tt_trace_opts *
     a.ld_sf = ld_sf;
     a.status = status;
     a.n_dev = n_dev;
+    a.forms = reinterpret_cast<const float4 *>(opts->forms);
+    TT_REQUIRE(!opts->forms || ((uintptr_t)opts->forms & 15) == 0, "tt_trace_rk4: forms must be 16-byte aligned");
     const int block = 256;
     TT_REQUIRE(opts->variant == 0 || opts->variant == 1, "tt_trace_rk4: variant must be 0 or 1");
     const dim3 nblk((unsigned)tt::ceil_div(n, block));
@@ -694,6 +811,8 @@ int trace_launch(const float *field4, const tt_grid *grid, const tt_trace_opts *
         hipLaunchKernelGGL((trace_rk4_kernel<false, false, true>), nblk, dim3(block), 0, st, a);
     else if (m2 && sf)
         hipLaunchKernelGGL((trace_rk4_kernel<true, true, true>), nblk, dim3(block), 0, st, a);
+    else if (m2 && a.forms)
+        hipLaunchKernelGGL((trace_rk4_kernel<true, true, false, true>), nblk, dim3(block), 0, st, a);
     else if (m2)
         hipLaunchKernelGGL((trace_rk4_kernel<true, true, false>), nblk, dim3(block), 0, st, a);
     else if (sf)

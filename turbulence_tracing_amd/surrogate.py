@@ -290,7 +290,9 @@ def build_dataset(n_fields, rays_per_field, cfg=None, seed=1000, device="cuda", 
     """Rays traced by the tracer through ``n_fields`` seeded fields: features
     of their cheap path estimate and the residual of the full trace."""
     cfg = cfg or Config()
-    cfg = Config(**{**cfg.__dict__, "n_rays": rays_per_field})
+    # few rays per field: the plane forms would not pay for their build (the
+    # trace's results are the same with or without them)
+    cfg = Config(**{**cfg.__dict__, "n_rays": rays_per_field, "plane_forms": False})
     p = Pipeline(cfg, device)
     feats, resids = [], []
     for f in range(n_fields):
