@@ -134,7 +134,13 @@ int tt_plane_forms(const float *field4, const tt_grid *grid, float *forms, void 
  * (e.g. from tt_ray_sort); outputs are written at each ray's own index.
  * sf (nullable) receives the state at c t = cT, continued straight past the
  * exit plane (the field is zero outside the grid, PT:239-241).
- * status (nullable, int32): 0 ok, 1 ray not moving forward in z, 2 NaN.
+ * status (nullable, int32): 0 ok; 1 handed over by the z-march: W = vz / c
+ * <= 0.97 at launch or at the end of a step (a slope above ~0.25, or a ray
+ * slowing towards a turning point, where a fixed z-step is not accurate to
+ * 1e-3 cells), or a side face (x or y) of the grid crossed (the field jumps to
+ * zero there); its outputs are NaN and tt_trace_time traces it in time;
+ * 2 NaN input (or a ray tt_trace_time gave up on); 3 traced in time by
+ * tt_trace_time.  Without status, handed-over rays just get NaN outputs.
  * variant 0: each lane caches its current cell as a trilinear polynomial in
  * registers (the fast path); variant 1: eight float4 gathers per RK4 stage.
  * With h = 1/2 and variant 0 the phase is summed by the composite Simpson
@@ -153,6 +159,24 @@ int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_trace_opts *
                  const double *s0, int64_t ld_s0, int64_t n, const int32_t *order,
                  double *rf, int64_t ld_rf, double *phase, double *sf, int64_t ld_sf,
                  int32_t *status, void *stream);
+
+/* Rays in time, as the reference integrates them (particle_tracking/
+ * particle_tracker.py:312-331, 398-419, then ray_at_exit PT:333-380), for the
+ * rays the z-march cannot follow: per ray, the Dormand-Prince 5(4) pair of
+ * the reference's RK45 in c t with its own step control at a tight tolerance
+ * (1e-10 of a cell in position, of c in velocity; fp64 state and trilinear
+ * weights, steps at most half the smallest cell edge, field zero outside the
+ * grid) from the launch state to c t = opts->cT, then the back-projection to
+ * the plane z = z.max() of ray_at_exit (so a ray that turned back is reported
+ * as the reference reports it).  With status non-NULL only the rays whose
+ * status equals only_status are traced (e.g. 1, the rays tt_trace_rk4 hands
+ * over), and their status becomes 3 (2 for NaN input,
+ * or for a ray given up after 2^18 attempted steps, with NaN outputs); with
+ * status NULL every ray is traced.  order, rf, phase, sf as in
+ * tt_trace_rk4; opts->h and variant are unused. */
+int tt_trace_time(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
+                  int64_t ld_s0, int64_t n, const int32_t *order, double *rf, int64_t ld_rf, double *phase,
+                  double *sf, int64_t ld_sf, int32_t *status, int32_t only_status, void *stream);
 
 /* tt_trace_rk4 over a ray list whose length is in device memory: the rays
  * order[0 .. *n_dev) are traced (n_dev: device int32, read by the kernel;

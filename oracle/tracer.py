@@ -30,6 +30,11 @@ from scipy.interpolate import RegularGridInterpolator
 
 C = sc.c                      # PT:119
 NC_COEFF = 3.14207787e-4      # PT:228, = eps0 m_e / e^2
+# The z-march hands a ray over to the time integrator below this W = vz / c
+# (a slope above ~0.25 or a ray slowing towards a turning point): beyond it
+# the fixed half-cell z-step misses 1e-3 cells (the GPU compares W - 1 with
+# -0.03 in float32).
+W_HANDOVER = 0.97
 
 
 def critical_density(lwl):
@@ -286,9 +291,15 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
     from its second step's start).  A partial first step and a leading odd
     step keep the RK4 weights.
 
+    Hand-over (status 1, outputs NaN, as the GPU's; the GPU then traces such
+    rays in time): W = vz / c - 1 <= W_HANDOVER - 1 at launch or at the end
+    of any step (steep rays, and rays slowing towards a turning point), or a
+    change of the ray's x / y inside-the-grid state between the lattice start
+    and any step end (a side face crossed: the field jumps to zero there).
+
     Returns rf (4 x N: x, theta, y, phi_angle), phase (N), sf (6 x N state at
     c t = cT, straight continuation past the exit plane), status (N; 0 ok,
-    1 non-forward ray, 2 NaN input)."""
+    1 handed over, 2 NaN input)."""
     nx, ny, nz = grid.shape
     x0, y0, z0 = grid.origin
     dx, dy, dz = grid.spacing
@@ -307,7 +318,7 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
     phase = np.zeros(N)
     status = np.zeros(N, dtype=np.int32)
     status[np.isnan(s0).any(axis=0)] = 2
-    status[(status == 0) & ~(W > 0)] = 1
+    status[(status == 0) & ~(W - 1.0 > W_HANDOVER - 1.0)] = 1
     ok = status == 0
 
     # straight carry to the first plane
@@ -352,6 +363,10 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
             hh = np.where(part, kstart * h - Zs, 0.0)
             y, dp, _ = rk4(Zs, y, hh, part)
             phase += dp
+        def lateral(yy):
+            return (yy[0] >= 0) & (yy[0] <= nx - 1) & (yy[1] >= 0) & (yy[1] <= ny - 1)
+
+        in0 = lateral(y)
         cellwise = m == 2
         kfirst = kstart + (kstart % 2) if cellwise else np.full(N, K + 1)
         pend = np.zeros(N)   # weight owed to the next step's start-point integrand
@@ -372,6 +387,13 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
                 continue
             ys0 = y
             y, dp, p1 = rk4(k * h, ys0, h, act)
+            # hand-over: W at or below W_HANDOVER, or a side face crossed
+            turned = act & (~(y[4] - 1.0 > W_HANDOVER - 1.0) | (lateral(y) != in0))
+            if turned.any():
+                status[turned] = 1
+                ok = ok & ~turned
+                act = act & ~turned
+                y = [np.where(turned, a_, b_) for a_, b_ in zip(ys0, y)]
             if not cellwise:
                 phase += dp
                 continue

@@ -296,6 +296,9 @@ def test_trace_null_and_slab_closed_forms(lib):
 
 
 def test_trace_nan_and_backward_rays_flagged(lib):
+    """A NaN ray stays NaN (status 2); a ray launched backwards is traced in
+    time (status 3) and, in the empty cube, reported by ray_at_exit's
+    back-projection of its straight path (PT:371-378)."""
     from turbulence_tracing_amd.particle_tracker import ElectronCube
     n = 9
     x = np.linspace(-5e-3, 5e-3, n)
@@ -308,8 +311,59 @@ def test_trace_nan_and_backward_rays_flagged(lib):
     cube.s0 = s0
     rf = cube.solve(verbose=False).cpu().numpy()
     st = cube.status.cpu().numpy()
-    assert st[1] == 2 and st[2] == 1 and (st[[0, 3, 4, 5, 6, 7]] == 0).all()
-    assert np.isnan(rf[:, 1]).all() and np.isnan(rf[:, 2]).all()
+    assert st[1] == 2 and st[2] == 3 and (st[[0, 3, 4, 5, 6, 7]] == 0).all()
+    assert np.isnan(rf[:, 1]).all()
+    s = s0.cpu().numpy()[:, 2]
+    t_bp = (s[2] - 5e-3) / s[5]  # straight path: back-projected from its start
+    np.testing.assert_allclose(rf[:, 2], [s[0] - s[3] * t_bp, np.arctan(s[3] / s[5]), s[1] - s[4] * t_bp,
+                                          np.arctan(s[4] / s[5])], rtol=1e-12, atol=1e-15)
+    assert cube.phase.cpu().numpy()[2] == 0.0
+
+
+def test_turning_rays_traced_in_time_match_tight_solve(lib):
+    """An overdense liner (test_liner beyond critical density, clipped at
+    ne_max = 1: PT:201-210, 230-231; the reference's notebook shows ray
+    angles reaching pi/2) turns rays back.  The z-march hands over the rays
+    that turn or come near turning (W <= 0.97) and tt_trace_time traces them
+    in time; both kinds, and the rays the z-march keeps, agree with tight
+    solve_ivp on the same cube: exit positions within 1e-3 cells, exit angles
+    within 1e-3 of the RMS deflection, phase within 1e-4 of its magnitude,
+    and the state at c t = cT (the reference's sol.y[:, -1]) within 1e-3
+    cells."""
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    n = 41
+    x = np.linspace(-5e-3, 5e-3, n)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.test_liner(n_e0=2e27, LR=1e-3)
+    cube.calc_dndr()
+    s0 = cube.init_beam(400, 3e-3, 1e-3, seed=8)
+    rf = cube.solve(verbose=False).cpu().numpy()
+    st = cube.status.cpu().numpy()
+    ph = cube.phase.cpu().numpy()
+    sf = cube.sf.cpu().numpy()
+    timed = np.flatnonzero(st == 3)
+    kept = np.flatnonzero(st == 0)
+    assert np.isin(st, (0, 3)).all()
+    assert np.isfinite(rf[:, timed]).all() and np.isfinite(ph[timed]).all()
+    back = timed[sf[5, timed] < 0]    # left through the entry side
+    fwd = timed[sf[5, timed] > 0]
+    assert back.size >= 10 and fwd.size >= 4 and kept.size >= 10
+    ref = OT.Cube(x, x.copy(), x.copy())
+    ref.test_liner(n_e0=2e27, LR=1e-3)
+    ref.calc_dndr()
+    ref.extent = ref.extent_z
+    sel = np.concatenate([back[:8], fwd[:4], kept[:4]])
+    rt, pt_, sft, _ = OT.solve_tight(ref, s0.cpu().numpy()[:, sel], bundle=False)
+    assert (sft[5, :8] < 0).all() and (sft[5, 8:] > 0).all()
+    dx = x[1] - x[0]
+    defl = np.sqrt(np.mean(rt[[1, 3]] ** 2))
+    pos = np.abs(rf[[0, 2]][:, sel] - rt[[0, 2]]) / dx
+    ang = np.abs(rf[[1, 3]][:, sel] - rt[[1, 3]])
+    phe = np.abs(ph[sel] - pt_) / np.abs(pt_).max()
+    print(f"liner: pos {pos.max():.2e} cells, angle {ang.max():.2e} rad of RMS {defl:.2e}, "
+          f"phase {phe.max():.2e}; time-traced back {back.size}, forward {fwd.size}; kept {kept.size}")
+    assert pos.max() <= 1e-3 and ang.max() <= 1e-3 * defl and phe.max() <= 1e-4
+    assert np.abs(sf[:3, sel] - sft[:3]).max() <= 1e-3 * dx
 
 
 def test_dndr_and_dsdt_match_reference_interpolator(lib, golden):

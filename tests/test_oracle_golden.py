@@ -297,6 +297,61 @@ def test_zmarch_matches_tight_solver_across_cell_faces():
     assert np.abs(ph - pt_).max() <= 1e-5 * np.abs(pt_).max()
 
 
+def test_zmarch_hands_over_steep_and_side_crossing_rays():
+    """The z-march's hand-over rule on the turbulent cube: rays launched at
+    0.3 rad (W = 0.955, where the half-cell z-step leaves 1e-3 of the
+    deflection) and rays that cross a side face of the grid (the field jumps
+    to zero there) get status 1; the rays it keeps match tight solve_ivp to
+    the bar of the path."""
+    n = 33
+    x, cube = turbulent_cube(n, 0.2)
+    dx = x[1] - x[0]
+    g = T.NodeGrid(cube)
+    steep = tilted_beam(6, -2e-3, 0.0, (0.3, 0.0), seed=5)
+    _, _, _, st = T.zmarch_rk4(g, steep)
+    assert (st == 1).all()
+    # 0.2 rad from x in [3, 4.5] mm: every ray leaves through x = 5 mm
+    side = tilted_beam(6, 3e-3, 4.5e-3, (0.2, 0.0), seed=6)
+    side[1] = np.linspace(-2e-3, 2e-3, 6)
+    _, _, _, st = T.zmarch_rk4(g, side)
+    assert (st == 1).all()
+    # kept: inside and at most 0.2 rad
+    keep = tilted_beam(8, -3e-3, 0.0, (0.2, 0.1), seed=7)
+    rf, ph, _, st = T.zmarch_rk4(g, keep)
+    assert (st == 0).all()
+    rt, pt_, _, _ = T.solve_tight(cube, keep, bundle=True)
+    launch = np.arctan(keep[[3, 4]] / keep[5])
+    defl = np.sqrt(np.mean((rt[[1, 3]] - launch) ** 2))
+    assert np.abs(rf[[0, 2]] - rt[[0, 2]]).max() <= 1e-3 * dx
+    assert np.abs(rf[[1, 3]] - rt[[1, 3]]).max() <= 1e-3 * defl
+
+
+def test_zmarch_keeps_paraxial_rays_of_an_overdense_liner():
+    """An overdense liner (test_liner at twice the critical density, clipped
+    at ne_max = 1; PT:201-210, 230-231) refracts by up to ~0.15 of a unit
+    velocity per cell: rays that come near turning are handed over, and the
+    ones the z-march keeps (W > 0.97 all the way) match tight solve_ivp
+    within 1e-3 cells and 1e-3 of the RMS deflection."""
+    n = 41
+    x = np.linspace(-5e-3, 5e-3, n)
+    dx = x[1] - x[0]
+    cube = T.Cube(x, x.copy(), x.copy())
+    cube.test_liner(n_e0=2e27, LR=1e-3)
+    cube.calc_dndr()
+    cube.extent = cube.extent_z
+    s0 = cube.init_beam(60, 3e-3, 1e-3, rng=np.random.RandomState(1))
+    rf, ph, _, st = T.zmarch_rk4(T.NodeGrid(cube), s0)
+    kept = np.flatnonzero(st == 0)
+    assert 10 <= kept.size <= 50      # both kinds present
+    rt, pt_, _, _ = T.solve_tight(cube, s0[:, kept], bundle=True)
+    defl = np.sqrt(np.mean(rt[[1, 3]] ** 2))
+    assert defl > 1e-2
+    assert np.abs(rf[[0, 2]][:, kept] - rt[[0, 2]]).max() <= 1e-3 * dx
+    assert np.abs(rf[[1, 3]][:, kept] - rt[[1, 3]]).max() <= 1e-3 * defl
+    # phase: 5.6e-5 of its magnitude on the most deflected kept ray (0.13 rad)
+    assert np.abs(ph[kept] - pt_).max() <= 1e-4 * np.abs(pt_).max()
+
+
 def test_zmarch_partial_first_step_and_outside_start():
     n = 9
     x = np.linspace(-5e-3, 5e-3, n)

@@ -165,6 +165,23 @@ __device__ __forceinline__ float phi_at(const TraceArgs &a, float pxb, float pyb
     return g.w * path_factor(s, W, iW);
 }
 
+// Hand-over rule of the z-march.  A ray leaves the march (status 1, traced
+// in time by tt_trace_time instead) when, at launch or at the end of any
+// step, W = vz / c <= 0.97 (a slope sqrt(U^2 + V^2) / W above ~0.25, or a ray
+// slowing in z towards a turning point), or when it crosses a side face of
+// the grid (x or y), where the field drops to zero and a fixed step across
+// the jump loses its order.  Below that W the fixed half-cell z-step is no
+// longer accurate to 1e-3 cells, and near W = 0 it can step over a turning
+// point.  With ne/nc clipped at 1 the field changes W by at most ~0.3 per
+// step, so a ray cannot pass from W > 0.97 to a turning point within one
+// step unnoticed.  The test costs nothing beyond the W > 0 test it replaces.
+constexpr float kWdMin = -0.03f;  // W - 1 at the hand-over
+
+__device__ __forceinline__ bool lateral_inside(const TraceArgs &a, float pxb, float pyb, const State &s) {
+    const float px = fmaf(a.sx, s.qx, pxb), py = fmaf(a.sy, s.qy, pyb);
+    return px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
+}
+
 // ---------------------------------------------------------------------------
 // Cell-cached evaluation.  A lane loads the cell its ray is in once, as the
 // trilinear polynomial f = a + b tx + c ty + d tz + e tx ty + f tx tz +
@@ -423,17 +440,21 @@ __device__ __forceinline__ void ledger_second(const TraceArgs &a, float pxb, flo
 // formed the same way on both routes, and the phase ledger follows each
 // lane's own steps, so what a lane computes never depends on the other lanes
 // of its wave).  A start between z-planes (odd k) takes one gathered step
-// first (its phase by the RK4 weights).
+// first (its phase by the RK4 weights).  Returns false if the ray is handed
+// over (the rule above kWdMin; the side faces are checked at each cell's start
+// and at the end).
 template <bool TAU, bool FORMS>
 __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
                                          double &ph) {
     bool good = true;
+    const bool in0 = lateral_inside(a, pxb, pyb, ys);  // x / y inside the grid at the start
+    bool crossed = false;                              // a side face crossed since
     if (k < K && (k & 1)) {
         ph += (double)rk4_step(a, pxb, pyb, (float)k * 0.5f, 0.5f, ys);
-        good = ys.Wd > -1.0f;
+        good = ys.Wd > kWdMin;
         ++k;
     }
-    if (k >= K) return good;
+    if (k >= K) return good && lateral_inside(a, pxb, pyb, ys) == in0;
     PhaseLedger L;
     float ox = 0.f, oy = 0.f;  // entry point relative to the current cell
     bool inside = false;       // the cell lies in the grid (x, y)
@@ -442,6 +463,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
     auto locate = [&](int &ix, int &iy) {
         const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
         inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
+        crossed = crossed || inside != in0;
         ix = inside ? min((int)px, a.nx - 2) : 0;
         iy = inside ? min((int)py, a.ny - 2) : 0;
         ox = pxb - (float)ix;
@@ -490,7 +512,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
                 ys = yn;
                 ledger_first(L, ph, phi, true, 0.f);
                 ++k;
-                good = good && (ys.Wd > -1.0f);
+                good = good && (ys.Wd > kWdMin);
                 const ZP z3 = zstep(zh, 0.25f, D);
                 const ZP z1 = zstep(z3, 0.25f, D);
                 yn = ys;
@@ -500,7 +522,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
                 ledger_second(a, pxb, pyb, k, L, ph, phi, true, 0.f, ys, yn);
                 ys = yn;
                 ++k;
-                good = good && (ys.Wd > -1.0f);
+                good = good && (ys.Wd > kWdMin);
                 if (k >= K) {
                     done = true;
                     break;
@@ -521,10 +543,10 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
         moved = !ok;
         ys = yn;
         ++k;
-        good = good && (ys.Wd > -1.0f);
+        good = good && (ys.Wd > kWdMin);
     }
     if (L.pend != 0.f) ph += (double)(L.pend * phi_at(a, pxb, pyb, (float)K * 0.5f, ys));  // exit point
-    return good;
+    return good && !crossed && lateral_inside(a, pxb, pyb, ys) == in0;
 }
 
 // Steps k .. K-1 (z = k h) from the lattice, cell-cached.  Every lane loads
@@ -534,16 +556,18 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
 // failing lanes redo the step with gathers, then every lane locates its cell
 // again.  The cached coefficients are dead inside that branch, so the gather
 // path does not add to the register footprint of the fast path.  Returns
-// false if the ray turned back (W <= 0).
+// false if the ray is handed over (the rule above kWdMin).
 template <bool TAU>
 __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, float pyb, int k, int K, State &ys,
                                              double &ph) {
     bool good = true;
+    const bool in0 = lateral_inside(a, pxb, pyb, ys);  // x / y inside the grid at the start
     while (k < K) {
         const int iz = k / a.m;
         const int kend = min(K, (iz + 1) * a.m);
         const float px = fmaf(a.sx, ys.qx, pxb), py = fmaf(a.sy, ys.qy, pyb);
         const bool inside = px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1;
+        good = good && inside == in0;
         const int ix = inside ? min((int)px, a.nx - 2) : 0;
         const int iy = inside ? min((int)py, a.ny - 2) : 0;
         Cell C;
@@ -564,7 +588,7 @@ __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, floa
             ys = yn;
             ph += (double)dph;
             ++k;
-            good = good && (ys.Wd > -1.0f);
+            good = good && (ys.Wd > kWdMin);
         }
         if (redo) {  // the cell is dead here; failing lanes gather
             if (!ok) {
@@ -574,10 +598,10 @@ __device__ __forceinline__ bool march_cached(const TraceArgs &a, float pxb, floa
             ys = yn;
             ph += (double)dph;
             ++k;
-            good = good && (ys.Wd > -1.0f);
+            good = good && (ys.Wd > kWdMin);
         }
     }
-    return good;
+    return good && lateral_inside(a, pxb, pyb, ys) == in0;
 }
 
 // One ray: the t-th of the launch (a.order[t] when given).  CACHED selects
@@ -592,7 +616,7 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
     int st = 0;
     if (isnan(x) || isnan(y) || isnan(z) || isnan(vx) || isnan(vy) || isnan(vz))
         st = 2;
-    else if (!(vz > 0.0))
+    else if (!(vz / tt::kC - 1.0 > (double)kWdMin))  // handed over at launch
         st = 1;
 
     const double U0 = vx / tt::kC, V0 = vy / tt::kC, W0 = vz / tt::kC;
@@ -630,9 +654,10 @@ __device__ __forceinline__ void trace_ray(const TraceArgs &a, int64_t t) {
                     st = 1;
             } else {
                 if (h0 > 1e-12) ph += (double)rk4_step(a, pxb, pyb, (float)Zs, (float)h0, ys);
+                const bool in0 = lateral_inside(a, pxb, pyb, ys);
                 for (int k = kstart; k < K; ++k) {
                     ph += (double)rk4_step(a, pxb, pyb, (float)k * a.h, a.h, ys);
-                    if (!(ys.Wd > -1.0f)) {  // turned back: z is no longer monotone
+                    if (!(ys.Wd > kWdMin) || lateral_inside(a, pxb, pyb, ys) != in0) {  // handed over
                         st = 1;
                         break;
                     }
@@ -707,6 +732,152 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(M2 ? 5 : 4)
         const int64_t s = chunk_start[c], e = min(s + (int64_t)chunk, a.n);
         for (int64_t t = s + threadIdx.x; t < e; t += blockDim.x) trace_ray<CACHED, M2, false>(a, t);
     }
+}
+
+// ------------------------------------------------------------ rays in time
+// Rays the z-march cannot follow (launched with vz <= 0, or turned back so
+// that W <= 0 on the way, status 1) are integrated as the reference does, in
+// time (PT:312-331, solve_ivp's RK45), but per ray and at a tight tolerance:
+// the Dormand-Prince 5(4) embedded pair (Dormand & Prince, J. Comput. Appl.
+// Math. 6 (1980) 19-26; the pair scipy's RK45 uses) in tau = c t on
+//   dX/dtau = U,  dU/dtau = g / dz,  dphase/dtau = (Phi / dz) |U|
+// (X in metres, U = v / c, g and Phi the node field's channels, which carry
+// one dz) from tau = 0 to cT.  A step is accepted when every component's
+// error estimate is within kTimeTol of its scale (dx, dy, dz for positions,
+// 1 for U, max(1, |phase|) for the phase); steps are at most half the
+// smallest cell edge and shrink where the trilinear field's kinks at cell
+// faces need it.  The state and the trilinear weights are fp64: these are few
+// rays, near a turning point W is small and its relative precision matters,
+// and fp32 cell coordinates would put noise of 1e-7 x n cells into the field
+// that a tolerance this tight would chase.  Once a ray is outside the grid
+// and moving away from it on that axis the field can no longer act, and it is
+// carried straight to cT.  A ray that needs more than kTimeMaxSteps attempted
+// steps is given up (NaN outputs, status 2).
+constexpr double kTimeTol = 1e-10;
+constexpr int kTimeMaxSteps = 1 << 18;
+
+__device__ __forceinline__ void time_rhs(const TraceArgs &a, const double (&y)[7], double (&k)[7]) {
+    const double px = (y[0] - a.x0) / a.dx, py = (y[1] - a.y0) / a.dy, pz = (y[2] - a.z0) / a.dz;
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    if (px >= 0.0 && px <= (double)a.fx1 && py >= 0.0 && py <= (double)a.fy1 && pz >= 0.0 && pz <= (double)a.fz1) {
+        const int ix = min((int)px, a.nx - 2), iy = min((int)py, a.ny - 2), iz = min((int)pz, a.nz - 2);
+        const double tx = px - ix, ty = py - iy, tz = pz - iz;
+        const int64_t plane = (int64_t)a.nx * a.ny;
+        const float4 *p = a.field + ((int64_t)iz * a.ny + iy) * a.nx + ix;
+        const int64_t off[8] = {0, 1, a.nx, a.nx + 1, plane, plane + 1, plane + a.nx, plane + a.nx + 1};
+        for (int c = 0; c < 8; ++c) {
+            const double w = ((c & 1) ? tx : 1.0 - tx) * ((c & 2) ? ty : 1.0 - ty) * ((c & 4) ? tz : 1.0 - tz);
+            const float4 v = p[off[c]];
+            g[0] = fma(w, (double)v.x, g[0]);
+            g[1] = fma(w, (double)v.y, g[1]);
+            g[2] = fma(w, (double)v.z, g[2]);
+            g[3] = fma(w, (double)v.w, g[3]);
+        }
+    }
+    const double idz = 1.0 / a.dz;
+    k[0] = y[3];
+    k[1] = y[4];
+    k[2] = y[5];
+    k[3] = g[0] * idz;
+    k[4] = g[1] * idz;
+    k[5] = g[2] * idz;
+    k[6] = g[3] * idz * sqrt(y[3] * y[3] + y[4] * y[4] + y[5] * y[5]);
+}
+
+// Dormand-Prince 5(4) tableau: stage nodes are implied by the rows; the last
+// row is the fifth-order solution (and its stage the next step's first, FSAL)
+__constant__ double kDPA[6][6] = {
+    {1.0 / 5, 0, 0, 0, 0, 0},
+    {3.0 / 40, 9.0 / 40, 0, 0, 0, 0},
+    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0, 0},
+    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0, 0},
+    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656, 0},
+    {35.0 / 384, 0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784, 11.0 / 84}};
+// fifth-order minus fourth-order weights, over the seven stages
+__constant__ double kDPE[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
+
+__global__ __launch_bounds__(256) void trace_time_kernel(TraceArgs a, int32_t only) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.n) return;
+    const int64_t r = a.order ? (int64_t)a.order[t] : t;
+    if (a.status && a.status[r] != only) return;
+    const double *s = a.s0 + r;
+    double y[7] = {s[0], s[a.ld_s0], s[2 * a.ld_s0], s[3 * a.ld_s0] / tt::kC, s[4 * a.ld_s0] / tt::kC,
+                   s[5 * a.ld_s0] / tt::kC, 0.0};
+    const double nanv = __longlong_as_double(0x7ff8000000000000ll);
+    bool bad = false;
+    for (int i = 0; i < 6; ++i) bad = bad || isnan(y[i]);
+    if (!bad) {
+        const double hmax = 0.5 * fmin(a.dx, fmin(a.dy, a.dz));
+        const double lo[3] = {a.x0, a.y0, a.z0};
+        const double hi[3] = {a.x0 + (double)a.fx1 * a.dx, a.y0 + (double)a.fy1 * a.dy, a.z0 + (double)a.fz1 * a.dz};
+        const double sc[6] = {kTimeTol * a.dx, kTimeTol * a.dy, kTimeTol * a.dz, kTimeTol, kTimeTol, kTimeTol};
+        double k[7][7];
+        time_rhs(a, y, k[0]);
+        double tau = 0.0, h = hmax;
+        int tries = 0;
+        while (tau < a.cT) {
+            bool gone = false;  // outside the grid and moving away from it
+            for (int d = 0; d < 3; ++d)
+                gone = gone || (y[d] < lo[d] && y[3 + d] <= 0.0) || (y[d] > hi[d] && y[3 + d] >= 0.0);
+            if (gone) break;
+            if (++tries > kTimeMaxSteps) {
+                bad = true;
+                break;
+            }
+            const double hh = fmin(h, a.cT - tau);
+            double yn[7];
+#pragma unroll
+            for (int st = 1; st <= 6; ++st) {
+#pragma unroll
+                for (int i = 0; i < 7; ++i) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int j = 0; j < st; ++j) acc = fma(kDPA[st - 1][j], k[j][i], acc);
+                    yn[i] = fma(hh, acc, y[i]);
+                }
+                time_rhs(a, yn, k[st]);
+            }
+            double en = 0.0;  // largest scaled error estimate
+#pragma unroll
+            for (int i = 0; i < 7; ++i) {
+                double e = 0.0;
+#pragma unroll
+                for (int j = 0; j < 7; ++j) e = fma(kDPE[j], k[j][i], e);
+                const double scale = i < 6 ? sc[i] : kTimeTol * fmax(1.0, fmax(fabs(y[6]), fabs(yn[6])));
+                en = fmax(en, fabs(hh * e) / scale);
+            }
+            double fac = 0.9 * pow(fmax(en, 1e-10), -0.2);
+            fac = fmin(5.0, fmax(0.2, fac));
+            if (en <= 1.0) {  // accept: the last stage is the next step's first
+#pragma unroll
+                for (int i = 0; i < 7; ++i) {
+                    y[i] = yn[i];
+                    k[0][i] = k[6][i];
+                }
+                tau += hh;
+            } else {
+                fac = fmin(fac, 1.0);
+            }
+            h = fmin(hh * fac, hmax);
+        }
+        if (!bad && tau < a.cT)  // straight to cT (no field acts any more)
+            for (int d = 0; d < 3; ++d) y[d] = fma(a.cT - tau, y[3 + d], y[d]);
+    }
+    // ray_at_exit (PT:333-380): back-project to the exit plane z = z.max()
+    const double ze = a.z0 + (double)a.fz1 * a.dz;
+    const double tbp = (y[2] - ze) / y[5];
+    double *rf = a.rf + r;
+    rf[0] = bad ? nanv : y[0] - y[3] * tbp;
+    rf[a.ld_rf] = bad ? nanv : atan(y[3] / y[5]);
+    rf[2 * a.ld_rf] = bad ? nanv : y[1] - y[4] * tbp;
+    rf[3 * a.ld_rf] = bad ? nanv : atan(y[4] / y[5]);
+    if (a.phase) a.phase[r] = bad ? nanv : y[6];
+    if (a.sf) {
+        double *sf = a.sf + r;
+        for (int i = 0; i < 6; ++i) sf[i * a.ld_sf] = bad ? nanv : (i < 3 ? y[i] : tt::kC * y[i]);
+    }
+    if (a.status) a.status[r] = bad ? 2 : 3;
 }
 
 // trilinear samples of the node field at arbitrary points (metres)
@@ -829,6 +1000,35 @@ extern "C" int tt_trace_rk4(const float *field4, const tt_grid *grid, const tt_t
                             double *phase, double *sf, int64_t ld_sf, int32_t *status, void *stream) {
     return trace_launch(field4, grid, opts, s0, ld_s0, n, order, nullptr, rf, ld_rf, phase, sf, ld_sf, status,
                         stream);
+}
+
+extern "C" int tt_trace_time(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
+                             int64_t ld_s0, int64_t n, const int32_t *order, double *rf, int64_t ld_rf, double *phase,
+                             double *sf, int64_t ld_sf, int32_t *status, int32_t only_status, void *stream) {
+    TT_REQUIRE(grid && opts, "tt_trace_time: null grid or options");
+    TT_REQUIRE(n >= 0, "tt_trace_time: n < 0");
+    if (n == 0) return TT_OK;  // an empty ray set: nothing is read or written
+    TT_REQUIRE(field4 && s0 && rf, "tt_trace_time: null pointer");
+    TT_REQUIRE(ld_s0 >= n && ld_rf >= n && (!sf || ld_sf >= n), "tt_trace_time: leading dimension < n");
+    TT_REQUIRE(opts->cT > 0.0, "tt_trace_time: cT must be > 0");
+    TraceArgs a{};
+    if (int rc = fill_grid_args(grid, a)) return rc;
+    a.field = reinterpret_cast<const float4 *>(field4);
+    a.cT = opts->cT;
+    a.s0 = s0;
+    a.ld_s0 = ld_s0;
+    a.n = n;
+    a.order = order;
+    a.rf = rf;
+    a.ld_rf = ld_rf;
+    a.phase = phase;
+    a.sf = sf;
+    a.ld_sf = ld_sf;
+    a.status = status;
+    hipLaunchKernelGGL(trace_time_kernel, dim3((unsigned)tt::ceil_div(n, 256)), dim3(256), 0, tt::as_stream(stream), a,
+                       only_status);
+    TT_LAUNCH_CHECK("tt_trace_time launch");
+    return TT_OK;
 }
 
 extern "C" int tt_trace_rk4_counted(const float *field4, const tt_grid *grid, const tt_trace_opts *opts,

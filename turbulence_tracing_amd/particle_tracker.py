@@ -15,7 +15,12 @@ What changes underneath:
     variable (h = 0.5 cells by default) instead of one adaptive RK45 over all
     rays.  It lands every ray on the exit plane that ray_at_exit projects to,
     and also returns the phase k0 int (n - 1) ds per ray (``self.phase``).
-    The ``method`` argument is accepted for signature compatibility.
+    Rays the z-march hands over (launched backwards or steeper than ~0.25,
+    slowed towards a turning point by an overdense region, or crossing a
+    side face of the grid) are then traced in time by tt_trace_time with
+    the reference's RK45 pair at a tight per-ray tolerance
+    (``self.status`` 3).  The ``method`` argument is accepted for signature
+    compatibility.
   * init_beam draws from numpy's global RNG exactly as the reference does
     (so seeded runs give the same rays) unless ``seed=`` is given, in which
     case the GPU Philox sampler draws the same distribution on the device.
@@ -295,6 +300,11 @@ class ElectronCube:
         self.lib.tt_trace_rk4(_lib.ptr(field4), _lib.C.byref(grid), _lib.C.byref(opts), _lib.ptr(s0), n, n,
                               _lib.ptr(order), _lib.ptr(self.rf), n, _lib.ptr(self.phase), _lib.ptr(sf), n,
                               _lib.ptr(self.status), _lib.stream_ptr())
+        # rays the z-march hands over (status 1) are traced in time, as the
+        # reference integrates every ray (status 3)
+        self.lib.tt_trace_time(_lib.ptr(field4), _lib.C.byref(grid), _lib.C.byref(opts), _lib.ptr(s0), n, n,
+                               _lib.ptr(order), _lib.ptr(self.rf), n, _lib.ptr(self.phase), _lib.ptr(sf), n,
+                               _lib.ptr(self.status), 1, _lib.stream_ptr())
         if self.probing_direction != "z":  # back to the cube's axes
             inv = [p.index(a) for a in range(3)]
             sf = sf[[inv[0], inv[1], inv[2], 3 + inv[0], 3 + inv[1], 3 + inv[2]]]

@@ -5,7 +5,8 @@ squares) -> node field (tt_grad_stencil with the unit-RMS normalisation and
 ne = ne0 (1 + eps delta) mapping applied on the device) and its plane forms
 (tt_plane_forms) -> beam
 (tt_beam_sample) -> ray order (tt_ray_sort, rays gathered into entry-column
-order) -> trace (tt_trace_rk4) -> shadowgraph, knife-edge schlieren and
+order) -> trace (tt_trace_rk4, then tt_trace_time for the rays it hands
+over) -> shadowgraph, knife-edge schlieren and
 interferogram histograms in one pass (tt_detect).
 
 This composes, on one device and without host round trips, what a user of
@@ -86,6 +87,7 @@ class Pipeline:
         self.sort_scratch = torch.empty(nbytes.value, dtype=torch.uint8, device=device)
         self.rf = torch.empty((4, N), dtype=f64, device=device)
         self.phase = torch.empty(N, dtype=f64, device=device)
+        self.status = torch.empty(N, dtype=torch.int32, device=device)
         P = cfg.pix
         self.H_shadow = torch.zeros((P, P), dtype=torch.int32, device=device)
         self.H_knife = torch.zeros((P, P), dtype=torch.int32, device=device)
@@ -147,11 +149,16 @@ class Pipeline:
                                   _lib.ptr(self.s0), N, _lib.stream_ptr())
 
     def trace(self, n=None):
+        """The z-march over rays [0, n), then the rays it hands over (status
+        1: steep, near turning, or crossing a side face) traced in time."""
         N = self.cfg.n_rays
         n = N if n is None else n
         self.lib.tt_trace_rk4(_lib.ptr(self.field4), _lib.C.byref(self.grid), _lib.C.byref(self.opts),
                               _lib.ptr(self.s0), N, n, None, _lib.ptr(self.rf), N, _lib.ptr(self.phase), None, N,
-                              None, _lib.stream_ptr())
+                              _lib.ptr(self.status), _lib.stream_ptr())
+        self.lib.tt_trace_time(_lib.ptr(self.field4), _lib.C.byref(self.grid), _lib.C.byref(self.opts),
+                               _lib.ptr(self.s0), N, n, None, _lib.ptr(self.rf), N, _lib.ptr(self.phase), None, N,
+                               _lib.ptr(self.status), 1, _lib.stream_ptr())
 
     def zero_histograms(self):
         self.H_shadow.zero_()
