@@ -35,6 +35,9 @@ NC_COEFF = 3.14207787e-4      # PT:228, = eps0 m_e / e^2
 # the fixed half-cell z-step misses 1e-3 cells (the GPU compares W - 1 with
 # -0.03 in float32).
 W_HANDOVER = 0.97
+# largest stage-to-stage relative change of W that the GPU's cell-cached step
+# takes (beyond it the step is redone by gathers, as a face crossing)
+NEWTON_TOL = 2.0 ** -12
 
 
 def critical_density(lwl):
@@ -283,9 +286,10 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
     The phase is summed with the RK4 weights of each step, except for
     h = 1/2, where it follows the GPU's cell-cached march: a cell [c, c + 1]
     whose two steps each keep all four stage points inside the x / y column
-    cell they start in takes Simpson's rule on the ray's points,
-    (f(c) + 4 f(c + 1/2) + f(c + 1)) / 6; a step whose stage points cross a
-    face takes the RK4 weights, and the other step of such a cell Simpson's
+    cell they start in (and W within NEWTON_TOL from stage to stage) takes
+    Simpson's rule on the ray's points, (f(c) + 4 f(c + 1/2) + f(c + 1)) / 6;
+    a step whose stage points cross a face (or whose W changes more) takes the
+    RK4 weights, and the other step of such a cell Simpson's
     rule on its half cell with the midpoint integrand taken a quarter cell
     back along the ray from that step's end (or, for the cell's first step,
     from its second step's start).  A partial first step and a leading odd
@@ -409,6 +413,11 @@ def zmarch_rk4(grid, s0, h=0.5, cT=None):
             for yy in (y2, y3, y4):
                 ex, ey = column_cell(yy)
                 smooth &= (ex == cx) & (ey == cy)
+            # the GPU refines 1/W from stage to stage by one Newton step and
+            # redoes a step exactly (as a face crossing) when W changed by
+            # more than 2^-12 between two stages
+            for ya, yb in ((ys0, y2), (y2, y3), (y3, y4)):
+                smooth &= np.abs(1.0 - yb[4] / ya[4]) <= NEWTON_TOL
             lead = act & (k < kfirst)          # a leading odd step: RK4
             phase += np.where(lead, dp, 0.0)
             cw = act & (k >= kfirst)

@@ -32,6 +32,7 @@ struct TraceArgs {
     const float4 *field;
     const float4 *forms;  // optional plane forms (tt_plane_forms): 4 float4 per node
     int nx, ny, nz;
+    uint32_t plane;       // nx ny (< 2^31)
     float fx1, fy1, fz1;  // n - 1 as float (upper bounds, inclusive)
     double x0, y0, z0, dx, dy, dz;
     float sx, sy;         // dz/dx, dz/dy
@@ -62,6 +63,14 @@ __device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
                        fmaf(t, b.w - a.w, a.w));
 }
 
+// Index of node (ix, iy, iz), all >= 0: iz (nx ny) + (iy nx + ix) as one
+// 32 x 32 -> 64-bit multiply-add and one 24-bit multiply-add (nx, ny < 2^24,
+// nx ny < 2^31), instead of a 64-bit multiply by nx.
+__device__ __forceinline__ int64_t node_index(const TraceArgs &a, int ix, int iy, int iz) {
+    const uint32_t row = __umul24((uint32_t)iy, (uint32_t)a.nx) + (uint32_t)ix;
+    return (int64_t)((uint64_t)(uint32_t)iz * a.plane + row);
+}
+
 // Trilinear sample in cell coordinates; zero strictly outside [0, n-1]^3.
 __device__ __forceinline__ float4 sample(const TraceArgs &a, float px, float py, float pz) {
     if (!(px >= 0.f && px <= a.fx1 && py >= 0.f && py <= a.fy1 && pz >= 0.f && pz <= a.fz1))
@@ -71,7 +80,7 @@ __device__ __forceinline__ float4 sample(const TraceArgs &a, float px, float py,
     const int iz = min((int)pz, a.nz - 2);
     const float tx = px - (float)ix, ty = py - (float)iy, tz = pz - (float)iz;
     const int64_t plane = (int64_t)a.nx * a.ny;
-    const float4 *p = a.field + ((int64_t)iz * a.ny + iy) * a.nx + ix;
+    const float4 *p = a.field + node_index(a, ix, iy, iz);
     const float4 c000 = p[0], c100 = p[1], c010 = p[a.nx], c110 = p[a.nx + 1];
     const float4 c001 = p[plane], c101 = p[plane + 1], c011 = p[plane + a.nx], c111 = p[plane + a.nx + 1];
     const float4 c00 = lerp4(c000, c100, tx), c10 = lerp4(c010, c110, tx);
@@ -204,7 +213,7 @@ __device__ __forceinline__ float4 f4fma(float4 p, float t, float4 q) {
 
 __device__ __forceinline__ void load_cell(const TraceArgs &A, int ix, int iy, int iz, Cell &C) {
     const int64_t plane = (int64_t)A.nx * A.ny;
-    const float4 *p = A.field + ((int64_t)iz * A.ny + iy) * A.nx + ix;
+    const float4 *p = A.field + node_index(A, ix, iy, iz);
     const float4 c000 = p[0], c100 = p[1], c010 = p[A.nx], c110 = p[A.nx + 1];
     const float4 c001 = p[plane], c101 = p[plane + 1], c011 = p[plane + A.nx], c111 = p[plane + A.nx + 1];
     C.a = c000;
@@ -289,18 +298,31 @@ struct ZP {
 // are r / W with r = (U, V, gx, gy, gz, -Wd) (s's velocity and the field
 // g), and 1 / W is returned in iW, so each use folds its RK4 weight into
 // the one factor c / W (one multiply) instead of scaling the five
-// derivatives first.
-template <bool VF>
+// derivatives first.  With SEED, iW comes in as the previous stage's 1 / W
+// and is refined by one Newton step instead of a v_rcp (which costs several
+// issue slots): its relative error e = 1 - W iW becomes e^2, within fp32
+// rounding while |e| <= 2^-12; e is returned so that a step whose W changed
+// more than that between stages is redone exactly.
+template <bool VF, bool SEED = false>
 __device__ __forceinline__ void rhs_z(const TraceArgs &a, const ZP &z, float ox, float oy, const State &s, float4 &g,
-                                      float &iW, float &vf, float &mn, float &mx) {
+                                      float &iW, float &vf, float &mn, float &mx, float &e) {
     const float tx = fmaf(a.sx, s.qx, ox), ty = fmaf(a.sy, s.qy, oy);
     mn = fminf(mn, fminf(tx, ty));
     mx = fmaxf(mx, fmaxf(tx, ty));
     g = f4fma(f4fma(z.E, ty, z.B), tx, f4fma(z.Cc, ty, z.A));
     const float W = 1.0f + s.Wd;
-    iW = __builtin_amdgcn_rcpf(W);
+    if (SEED) {
+        e = fmaf(-W, iW, 1.0f);
+        iW = fmaf(iW, e, iW);
+    } else {
+        iW = __builtin_amdgcn_rcpf(W);
+    }
     if (VF) vf = path_factor(s, W, iW);
 }
+
+// the largest stage-to-stage relative change of W a cached step may have
+// (the Newton-refined 1 / W is then exact to fp32 rounding)
+constexpr float kNewtonTolInv = 4096.0f;
 
 // y + c r, r the unscaled right-hand side of stage state s with field g
 template <bool TAU>
@@ -314,31 +336,33 @@ __device__ __forceinline__ State axpy_r(const State &y, float c, const State &s,
 // phase is summed by the composite Simpson rule over the march's lattice
 // points (march_m2), so only the first stage needs the Phi channel and the
 // others' Phi (and the Phi channel of the zm and z1 planes) is never formed.
+// mn / mx collect the stage points' in-cell coordinates, and mx also
+// 4096 x the largest relative change of W between stages: the step is good
+// from the cache only if mn >= 0 and mx <= 1.  iWc: in, 1 / W of a state near
+// the start (the previous step's last stage, or exact); out, the last
+// stage's 1 / W, the next step's seed.
 template <bool TAU>
 __device__ __forceinline__ float rk4_half_z(const TraceArgs &a, const ZP &z0, const ZP &zm, const ZP &z1, float ox,
-                                            float oy, State &y, float &mn, float &mx) {
-    const State zero{0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                                            float oy, State &y, float &mn, float &mx, float &iWc) {
+    // the new state is accumulated as y + h/6 (k1 + 2 k2 + 2 k3 + k4) stage
+    // by stage, each weight folded into that stage's factor c / W
+    constexpr float h6 = 0.5f / 6.0f, h3 = 0.5f / 3.0f;
     float4 g;
-    float iW, vf = 0.f, unused;
-    rhs_z<true>(a, z0, ox, oy, y, g, iW, vf, mn, mx);
+    float iW = iWc, vf = 0.f, unused, e1, e2, e3, e4;
+    rhs_z<true, true>(a, z0, ox, oy, y, g, iW, vf, mn, mx, e1);
     const float phi = g.w * vf;
-    State acc = axpy_r<TAU>(zero, iW, y, g);  // k1 + 2 k2 + 2 k3 + k4, k = r / W
+    State yn = axpy_r<TAU>(y, h6 * iW, y, g);
     const State s2 = axpy_r<TAU>(y, 0.25f * iW, y, g);
-    rhs_z<false>(a, zm, ox, oy, s2, g, iW, unused, mn, mx);
-    acc = axpy_r<TAU>(acc, 2.f * iW, s2, g);
+    rhs_z<false, true>(a, zm, ox, oy, s2, g, iW, unused, mn, mx, e2);
+    yn = axpy_r<TAU>(yn, h3 * iW, s2, g);
     const State s3 = axpy_r<TAU>(y, 0.25f * iW, s2, g);
-    rhs_z<false>(a, zm, ox, oy, s3, g, iW, unused, mn, mx);
-    acc = axpy_r<TAU>(acc, 2.f * iW, s3, g);
+    rhs_z<false, true>(a, zm, ox, oy, s3, g, iW, unused, mn, mx, e3);
+    yn = axpy_r<TAU>(yn, h3 * iW, s3, g);
     const State s4 = axpy_r<TAU>(y, 0.5f * iW, s3, g);
-    rhs_z<false>(a, z1, ox, oy, s4, g, iW, unused, mn, mx);
-    acc = axpy_r<TAU>(acc, iW, s4, g);
-    const float h6 = 0.5f * (1.0f / 6.0f);
-    y.qx = fmaf(h6, acc.qx, y.qx);
-    y.qy = fmaf(h6, acc.qy, y.qy);
-    y.U = fmaf(h6, acc.U, y.U);
-    y.V = fmaf(h6, acc.V, y.V);
-    y.Wd = fmaf(h6, acc.Wd, y.Wd);
-    if (TAU) y.tr = fmaf(h6, acc.tr, y.tr);
+    rhs_z<false, true>(a, z1, ox, oy, s4, g, iW, unused, mn, mx, e4);
+    y = axpy_r<TAU>(yn, h6 * iW, s4, g);
+    iWc = iW;
+    mx = fmaxf(mx, kNewtonTolInv * fmaxf(fmaxf(fabsf(e1), fabsf(e2)), fmaxf(fabsf(e3), fabsf(e4))));
     return phi;
 }
 
@@ -351,14 +375,14 @@ template <bool FORMS>
 __device__ __forceinline__ void load_planes(const TraceArgs &A, int ix, int iy, int iz, ZP &z0, ZP &D) {
     if (FORMS) {
         const int64_t plane4 = 4 * (int64_t)A.nx * A.ny;
-        const float4 *p = A.forms + 4 * (((int64_t)iz * A.ny + iy) * A.nx + ix);
+        const float4 *p = A.forms + 4 * node_index(A, ix, iy, iz);
         z0 = ZP{p[0], p[1], p[2], p[3]};
         const float4 a1 = p[plane4], b1 = p[plane4 + 1], c1 = p[plane4 + 2], e1 = p[plane4 + 3];
         D = ZP{f4sub(a1, z0.A), f4sub(b1, z0.B), f4sub(c1, z0.Cc), f4sub(e1, z0.E)};
         return;
     }
     const int64_t plane = (int64_t)A.nx * A.ny;
-    const float4 *p = A.field + ((int64_t)iz * A.ny + iy) * A.nx + ix;
+    const float4 *p = A.field + node_index(A, ix, iy, iz);
     const float4 c000 = p[0], c100 = p[1], c010 = p[A.nx], c110 = p[A.nx + 1];
     const float4 c001 = p[plane], c101 = p[plane + 1], c011 = p[plane + A.nx], c111 = p[plane + A.nx + 1];
     z0.A = c000;
@@ -456,6 +480,10 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
     }
     if (k >= K) return good && lateral_inside(a, pxb, pyb, ys) == in0;
     PhaseLedger L;
+    // 1 / W seeding each cached step's Newton refinement: the previous step's
+    // last stage, or exact after a gathered step (per lane, so that a lane's
+    // results never depend on which other lanes of its wave redid a step)
+    float iWc = __builtin_amdgcn_rcpf(1.0f + ys.Wd);
     float ox = 0.f, oy = 0.f;  // entry point relative to the current cell
     bool inside = false;       // the cell lies in the grid (x, y)
     bool moved = false;        // this lane's last step was redone because it left its cell
@@ -488,7 +516,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
             const ZP z3 = zstep(zh, 0.25f, D);
             const ZP z1 = zstep(z3, 0.25f, D);
             yn = ys;
-            phi = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
+            phi = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx, iWc);
             ok = mn >= 0.f && mx <= 1.f;
         } else {  // whole cells from the cache
             bool done = false;
@@ -506,7 +534,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
                 const ZP zq = zstep(z0, 0.25f, D);
                 const ZP zh = zstep(z0, 0.5f, D);
                 yn = ys;
-                phi = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx);
+                phi = rk4_half_z<TAU>(a, z0, zq, zh, ox, oy, yn, mn, mx, iWc);
                 ok = mn >= 0.f && mx <= 1.f;
                 if (__ballot(!ok) != 0) break;  // redo the cell's first step below
                 ys = yn;
@@ -516,7 +544,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
                 const ZP z3 = zstep(zh, 0.25f, D);
                 const ZP z1 = zstep(z3, 0.25f, D);
                 yn = ys;
-                phi = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx);
+                phi = rk4_half_z<TAU>(a, zh, z3, z1, ox, oy, yn, mn, mx, iWc);
                 ok = mn >= 0.f && mx <= 1.f;
                 if (__ballot(!ok) != 0) break;  // redo the cell's second step below
                 ledger_second(a, pxb, pyb, k, L, ph, phi, true, 0.f, ys, yn);
@@ -535,6 +563,7 @@ __device__ __forceinline__ bool march_m2(const TraceArgs &a, float pxb, float py
         if (!ok) {
             yn = ys;
             phi = rk4_step_phi(a, pxb, pyb, (float)k * 0.5f, 0.5f, yn, dph);
+            iWc = __builtin_amdgcn_rcpf(1.0f + yn.Wd);  // an exact seed after a gathered step
         }
         if (k & 1)
             ledger_second(a, pxb, pyb, k, L, ph, phi, ok, dph, ys, yn);
@@ -763,7 +792,7 @@ __device__ __forceinline__ void time_rhs(const TraceArgs &a, const double (&y)[7
         const int ix = min((int)px, a.nx - 2), iy = min((int)py, a.ny - 2), iz = min((int)pz, a.nz - 2);
         const double tx = px - ix, ty = py - iy, tz = pz - iz;
         const int64_t plane = (int64_t)a.nx * a.ny;
-        const float4 *p = a.field + ((int64_t)iz * a.ny + iy) * a.nx + ix;
+        const float4 *p = a.field + node_index(a, ix, iy, iz);
         const int64_t off[8] = {0, 1, a.nx, a.nx + 1, plane, plane + 1, plane + a.nx, plane + a.nx + 1};
         for (int c = 0; c < 8; ++c) {
             const double w = ((c & 1) ? tx : 1.0 - tx) * ((c & 2) ? ty : 1.0 - ty) * ((c & 4) ? tz : 1.0 - tz);
@@ -906,9 +935,12 @@ __global__ __launch_bounds__(256) void sample_kernel(TraceArgs a, const double *
 int fill_grid_args(const tt_grid *grid, TraceArgs &a) {
     TT_REQUIRE(grid->nx >= 2 && grid->ny >= 2 && grid->nz >= 2, "grid needs >= 2 nodes per axis");
     TT_REQUIRE(grid->dx > 0 && grid->dy > 0 && grid->dz > 0, "grid spacing must be > 0");
+    TT_REQUIRE((int64_t)grid->nx * grid->ny < ((int64_t)1 << 31), "grid: nx * ny must be < 2^31, got %d x %d",
+               grid->nx, grid->ny);
     a.nx = grid->nx;
     a.ny = grid->ny;
     a.nz = grid->nz;
+    a.plane = (uint32_t)grid->nx * (uint32_t)grid->ny;
     a.fx1 = (float)(grid->nx - 1);
     a.fy1 = (float)(grid->ny - 1);
     a.fz1 = (float)(grid->nz - 1);
