@@ -77,11 +77,20 @@ __device__ __forceinline__ double small_atan(double x) {
 
 struct PathState {
     float dpx, dpy, U, V, Wd;
+    float iW;  // 1 / (1 + Wd), carried from the step that set Wd
     double ph;
     float segx, segy, crossx, crossy, qx, qy;
     int cix, ciy;
     int seg, seg_end;
 };
+
+// 1 / W by two Newton steps from a seed y near it
+__device__ __forceinline__ float newton_rcp2(float W, float y) {
+    float e = fmaf(-W, y, 1.0f);
+    y = fmaf(y, e, y);
+    e = fmaf(-W, y, 1.0f);
+    return fmaf(y, e, y);
+}
 
 // Mid-step position of a step of width w from the current state (cells),
 // its column (0 with inb = false outside the x/y range) and fractions.
@@ -94,7 +103,7 @@ struct Mid {
 __device__ __forceinline__ Mid half_drift(const PathArgs &a, const PathState &S, float pxb, float pyb, float w) {
     Mid m;
     m.W = 1.0f + S.Wd;
-    m.iW = __builtin_amdgcn_rcpf(m.W);
+    m.iW = S.iW;
     const float hm = 0.5f * w;
     m.dmx = fmaf(hm * a.sx * S.U, m.iW, S.dpx);
     m.dmy = fmaf(hm * a.sy * S.V, m.iW, S.dpy);
@@ -113,14 +122,20 @@ __device__ __forceinline__ Mid half_drift(const PathArgs &a, const PathState &S,
 __device__ __forceinline__ void kick_drift(const PathArgs &a, const Mid &m, float4 g, float w, float mvx, float mvy,
                                            int c0, int ncell, PathState &S, float *f) {
     const float hm = 0.5f * w;
-    // W at the step's middle: g.z is the whole step's change of W (times W)
-    const float iWm = __builtin_amdgcn_rcpf(fmaf(0.5f * g.z, m.iW, m.W));
+    // W at the step's middle: g.z is the whole step's change of W (times W).
+    // Its reciprocal, and the step end's, by two Newton steps each from the
+    // nearby 1 / W (relative error e -> e^4: exact to fp32 rounding while W
+    // changes by less than 1.5% over half a step) instead of v_rcp, which
+    // costs several issue slots
+    const float Wm = fmaf(0.5f * g.z, m.iW, m.W);
+    float iWm = newton_rcp2(Wm, m.iW);
     const float sp = __builtin_amdgcn_sqrtf(fmaf(S.U, S.U, fmaf(S.V, S.V, m.W * m.W))) * m.iW;
     S.ph += (double)(sp * g.w);
     S.U = fmaf(g.x, iWm, S.U);
     S.V = fmaf(g.y, iWm, S.V);
     S.Wd = fmaf(g.z, iWm, S.Wd);
-    const float iW1 = __builtin_amdgcn_rcpf(1.0f + S.Wd);
+    const float iW1 = newton_rcp2(1.0f + S.Wd, iWm);
+    S.iW = iW1;
     S.dpx = fmaf(fmaf(hm, S.U, -mvx) * a.sx, iW1, m.dmx);
     S.dpy = fmaf(fmaf(hm, S.V, -mvy) * a.sy, iW1, m.dmy);
     // z-segment sums: segment q holds cells [ceil(q ncell / 16), ceil((q + 1) ncell / 16))
@@ -278,6 +293,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NODE ? 5 : 
     S.U = (float)U0;
     S.V = (float)V0;
     S.Wd = (float)(W0 - 1.0);
+    S.iW = __builtin_amdgcn_rcpf(1.0f + S.Wd);
     S.ph = 0.0;
     S.segx = S.segy = S.crossx = S.crossy = S.qx = S.qy = 0.f;
     S.cix = S.ciy = -1;
