@@ -366,6 +366,47 @@ def test_turning_rays_traced_in_time_match_tight_solve(lib):
     assert np.abs(sf[:3, sel] - sft[:3]).max() <= 1e-3 * dx
 
 
+def test_trace_where_w_changes_fast_between_stages(lib):
+    """A steep z-ramp of the density (about four cells wide, up to ne/nc =
+    0.05, so every ray keeps W > 0.97): W changes by ~2e-3 between RK4 stages
+    there, more than the 2^-12 within which the cached step's Newton-refined
+    1/W is exact, so those steps are redone by gathers (and summed by the RK4
+    weights).  The GPU agrees with the FP64 z-march that follows the same
+    rule, to the bar and to 1e-5 of the phase, and with tight solve_ivp."""
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
+    n = 41
+    x = np.linspace(-5e-3, 5e-3, n)
+    dx = x[1] - x[0]
+    XX, _, ZZ = np.meshgrid(x, x, x, indexing="ij")
+    nc = OT.critical_density(1053e-9)
+    ne = nc * 0.05 * 0.5 * (1 + np.tanh(ZZ / (2 * dx))) * (1 + 0.3 * XX / 5e-3)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne)
+    cube.calc_dndr()
+    s0 = cube.init_beam(2000, 2e-3, 1e-3, seed=21)
+    rf = cube.solve(verbose=False).cpu().numpy()
+    ph = cube.phase.cpu().numpy()
+    assert (cube.status.cpu().numpy() == 0).all()
+    ref = OT.Cube(x, x.copy(), x.copy())
+    ref.external_ne(ne)
+    ref.calc_dndr()
+    ref.extent = ref.extent_z
+    s = s0.cpu().numpy()
+    rf_o, ph_o, _, st = OT.zmarch_rk4(OT.NodeGrid(ref), s)
+    assert (st == 0).all()
+    launch = np.arctan(s[[3, 4]] / s[5])
+    defl = np.sqrt(np.mean((rf_o[[1, 3]] - launch) ** 2))
+    assert defl > 1e-3
+    assert np.abs(rf[[0, 2]] - rf_o[[0, 2]]).max() <= 1e-3 * dx
+    assert np.abs(rf[[1, 3]] - rf_o[[1, 3]]).max() <= 1e-3 * defl
+    assert np.abs(ph - ph_o).max() <= 1e-5 * np.abs(ph_o).max()
+    sel = np.arange(0, 2000, 250)
+    rt, pt_, _, _ = OT.solve_tight(ref, s[:, sel], bundle=True)
+    assert np.abs(rf[[0, 2]][:, sel] - rt[[0, 2]]).max() <= 1e-3 * dx
+    assert np.abs(rf[[1, 3]][:, sel] - rt[[1, 3]]).max() <= 1e-3 * defl
+    assert np.abs(ph[sel] - pt_).max() <= 1e-5 * np.abs(pt_).max()
+
+
 def test_dndr_and_dsdt_match_reference_interpolator(lib, golden):
     from turbulence_tracing_amd import particle_tracker as pt
     x = golden["cube_x"]
