@@ -362,11 +362,22 @@ __device__ __forceinline__ void pipe_layer(const Pipe<G> &P, const float *bias, 
             // raised wave priority while this wave issues its MFMA chain, so the
             // SIMD's other waves' epilogues fill the gaps rather than delay it
             __builtin_amdgcn_s_setprio(1);
+            // the fragments in halves of up to four: a half's LDS reads are all
+            // issued before its MFMAs (sched_barrier keeps the scheduler from
+            // sinking each read to its use, which left two reads per wait)
+            constexpr int HALF = KC < 4 ? KC : 4;
 #pragma unroll
-            for (int cc = 0; cc < KC; ++cc) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(lb + ((tl * KC + cc) * 64 + lane) * 8);
+            for (int c0 = 0; c0 < KC; c0 += HALF) {
+                bf16x8 af[HALF];
 #pragma unroll
-                for (int t = 0; t < T; ++t) acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, x[cc][t], acc[t], 0, 0, 0);
+                for (int j = 0; j < HALF; ++j)
+                    af[j] = *reinterpret_cast<const bf16x8 *>(lb + ((tl * KC + c0 + j) * 64 + lane) * 8);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int j = 0; j < HALF; ++j)
+#pragma unroll
+                    for (int t = 0; t < T; ++t)
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[j], x[c0 + j][t], acc[t], 0, 0, 0);
             }
             __builtin_amdgcn_s_setprio(0);
             if (GELU) {
