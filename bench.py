@@ -174,6 +174,7 @@ def main():
     if base is not None:
         try:
             out["cpu_baseline"] = cpu_baseline(base, p, cfg, args.cpu_bundle, args.cpu_budget, out["rays_per_s"])
+            out["cpu_baseline"]["field"] = cpu_field_baseline(cfg, base.P, stages["field"])
         finally:
             base.close()
     print(json.dumps(out), flush=True)
@@ -612,6 +613,56 @@ def cpu_baseline(base, p, cfg, bundle, budget, gpu_rays_per_s):
                   f"GPU's problem (rays x {cfg.steps_per_ray}); the as-shipped mode ({modes['as_shipped']['rays']} "
                   f"rays, one RK45 per {bundle}-ray bundle at rtol 1e-3) is in modes",
         "modes": modes,
+    }
+
+
+def cpu_field_baseline(cfg, workers, gpu_field_ms):
+    """The field on the host cores, as the reference makes it: the oracle's
+    restatement of gaussian3D_FFT (TG:488-538: MT19937 normal draws, the
+    flips, k_func, numpy's single-thread ifftn) at N = 121 (243^3 = 3^15, an
+    FFT-friendly size of the reference's odd 2N + 1 family), and the
+    transform alone at 256^3 with numpy (one thread) and scipy.fft (workers =
+    the CPU pool's size), each the better of two runs and extrapolated to
+    the benchmark's grid by n^3 log n (stated as extrapolated: at 512^3 one
+    reference field takes ~50 s of one core)."""
+    import numpy as np
+    import scipy.fft as sfft
+    from oracle import field as F
+
+    def k_func(K):
+        with np.errstate(divide="ignore"):
+            P = K ** (-11.0 / 3.0)
+        P[(K < 1.0 / 243) | (K > 0.5)] = 0.0
+        return P
+
+    def scale(m):
+        n = cfg.n
+        return (n ** 3 * np.log(n ** 3)) / (m ** 3 * np.log(m ** 3))
+
+    def best_of_two(fn):  # the first call also pays for faulting in fresh memory
+        ts = []
+        for _ in range(2):
+            t = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t)
+        return min(ts)
+
+    ref_s = best_of_two(lambda: F.gaussian3D_FFT(121, k_func, rng=np.random.RandomState(0)))
+    a = np.random.RandomState(1).randn(256, 256, 256) + 0j
+    np_s = best_of_two(lambda: np.fft.ifftn(a))
+    sp_s = best_of_two(lambda: sfft.ifftn(a, workers=workers))
+    del a
+    ref_ext = ref_s * scale(243)
+    log(f"cpu field: gaussian3D_FFT 243^3 {ref_s:.2f} s, ifftn 256^3 numpy {np_s:.2f} s, scipy x{workers} {sp_s:.2f} s")
+    return {
+        "reference_243_s": round(ref_s, 3), "numpy_ifftn_256_s": round(np_s, 3),
+        f"scipy_ifftn_256_workers{workers}_s": round(sp_s, 3),
+        f"extrapolated_{cfg.n}": {"reference_s": round(ref_ext, 2), "numpy_ifftn_s": round(np_s * scale(256), 2),
+                                  "scipy_ifftn_s": round(sp_s * scale(256), 2)},
+        "gpu_field_ms": gpu_field_ms,
+        "gpu_over_reference": ref_ext / (gpu_field_ms / 1e3),
+        "note": "reference = the oracle's restatement of gaussian3D_FFT (MT19937 draws, flips, k_func, "
+                "single-thread numpy ifftn); extrapolated by n^3 log n",
     }
 
 
