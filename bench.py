@@ -623,8 +623,8 @@ def cpu_field_baseline(cfg, workers, gpu_field_ms):
     FFT-friendly size of the reference's odd 2N + 1 family), and the
     transform alone at 256^3 with numpy (one thread) and scipy.fft (workers =
     the CPU pool's size), each the better of two runs and extrapolated to
-    the benchmark's grid by n^3 log n (stated as extrapolated: at 512^3 one
-    reference field takes ~50 s of one core)."""
+    the benchmark's grid by n^3 log n (stated as extrapolated: a reference
+    field at 512^3 takes seconds to tens of seconds of one core)."""
     import numpy as np
     import scipy.fft as sfft
     from oracle import field as F
