@@ -103,7 +103,7 @@ __device__ __forceinline__ void rhs(const TraceArgs &a, float pxb, float pyb, fl
                                     float &gw, float &vf) {
     const float4 g = sample(a, fmaf(a.sx, s.qx, pxb), fmaf(a.sy, s.qy, pyb), Z);
     const float W = 1.0f + s.Wd;
-    const float iW = 1.0f / W;
+    const float iW = __builtin_amdgcn_rcpf(W);
     k.qx = s.U * iW;
     k.qy = s.V * iW;
     k.U = g.x * iW;
@@ -168,7 +168,7 @@ __device__ __forceinline__ float rk4_step_phi(const TraceArgs &a, float pxb, flo
 __device__ __forceinline__ float phi_at(const TraceArgs &a, float pxb, float pyb, float Z, const State &s,
                                         float back = 0.f) {
     const float W = 1.0f + s.Wd;
-    const float iW = 1.0f / W;
+    const float iW = __builtin_amdgcn_rcpf(W);
     const float qx = fmaf(-back * s.U, iW, s.qx), qy = fmaf(-back * s.V, iW, s.qy);
     const float4 g = sample(a, fmaf(a.sx, qx, pxb), fmaf(a.sy, qy, pyb), Z);
     return g.w * path_factor(s, W, iW);
