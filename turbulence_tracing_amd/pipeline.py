@@ -47,9 +47,10 @@ class Config:
     variant: int = 0             # tracer kernel: 0 cell-cached, 1 per-stage gathers
     brick_shift: int = 0         # ray order: entry columns grouped in bricks of 2^shift cells
     # build the plane-form field (64 B per node, tt_plane_forms) for the
-    # tracer's cell cache: it saves ~6% of the trace's instructions and costs
-    # one pass writing 4x the node field, so it pays when many rays cross one
-    # field (the large volume's waves), not for one 1e7-ray trace of a 512^3 field
+    # tracer's cell cache: it saves ~10% of the trace's instructions (187
+    # instead of 210 per wave-step) but costs a pass writing 4x the node field
+    # and the trace's loads of it; measured slower overall both for a 1e7-ray
+    # trace of a 512^3 field and for the 1024^3 waves, hence off by default
     plane_forms: bool = False
 
     @property
