@@ -194,7 +194,11 @@ int tt_trace_rk4_counted(const float *field4, const tt_grid *grid, const tt_trac
  * replaces running ElectronCube.solve once per realisation, as the reference's
  * drivers do in a loop or across MPI ranks (particle_tracking/
  * example_multiprocess.py:41-62, example_MPI.py:82-149).  Each item is one
- * tt_trace_rk4 call's arguments (no sf / status).  tt_batch_prepare uploads the
+ * tt_trace_rk4 call's arguments (no sf).  With a status array, the rays the
+ * z-march hands over (status 1) are then traced in time, as tt_trace_time
+ * does after tt_trace_rk4 (light-travel budget cT, m, as in tt_trace_opts),
+ * in a second launch of the same call; without one they are NaN.
+ * tt_batch_prepare uploads the
  * items and a chunk table (chunk rays per entry, items with the most z-steps
  * first) into `workspace` with synchronous copies and fills `info`;
  * tt_batch_trace is stream-ordered and allocation-free, so it can be captured
@@ -210,6 +214,8 @@ typedef struct {
     double *rf;
     int64_t ld_rf;
     double *phase;
+    int32_t *status; /* per ray as tt_trace_rk4's, or NULL */
+    double cT;       /* used with status: > 0 */
 } tt_batch_item;
 
 typedef struct {

@@ -1,5 +1,6 @@
 """Batched ensembles: the persistent multi-realisation trace against one
-tt_trace_rk4 call per member (bit for bit, every launch shape), the captured
+tt_trace_rk4 call per member (bit for bit, every launch shape, rays handed
+over to the time integrator included), the captured
 ensemble step against the eager one over several replays, and the autotuner's
 choice and cache."""
 import json
@@ -68,6 +69,41 @@ def test_batch_trace_matches_per_member(lib, variant):
             np.testing.assert_array_equal(p.phase.cpu().numpy(), ph)
 
 
+def test_batch_trace_hands_over_like_per_member(lib):
+    """Every 7th ray of every member launched at 0.3 rad (W = 0.954): the
+    z-march hands them over (status 1) and the batched call traces them in
+    time, as tt_trace_rk4 + tt_trace_time do per member: rf, phase and status
+    bit for bit, over two launch shapes."""
+    from turbulence_tracing_amd.ensemble import Ensemble
+    from turbulence_tracing_amd.particle_tracker import c
+    ens = Ensemble(small_members(), pix=64)
+    ens.generate()
+    ens.order()
+    for p in ens.pipes:
+        p.s0[3, ::7] = 0.3 * c
+        p.s0[4, ::7] = 0.0
+        p.s0[5, ::7] = float(np.sqrt(1.0 - 0.09)) * c
+    ref = []
+    for p in ens.pipes:
+        p.trace()
+        ref.append((p.rf.cpu().numpy().copy(), p.phase.cpu().numpy().copy(), p.status.cpu().numpy().copy()))
+    for rf, ph, st in ref:
+        assert (st[::7] == 3).all() and np.isfinite(rf[:, ::7]).all(), "steep rays traced in time"
+        assert set(np.unique(st).tolist()) <= {0, 3}
+    for block, bpc, chunk in [(256, 0, 4096), (128, 2, 64)]:
+        for p in ens.pipes:
+            p.rf.fill_(7.0)
+            p.phase.fill_(7.0)
+            p.status.fill_(-1)
+        ens.tuning = {"block": block, "blocks_per_cu": bpc, "variant": 0, "chunk": chunk, "brick_shift": 0}
+        ens.trace()
+        torch.cuda.synchronize()
+        for (rf, ph, st), p in zip(ref, ens.pipes):
+            np.testing.assert_array_equal(p.rf.cpu().numpy(), rf)
+            np.testing.assert_array_equal(p.phase.cpu().numpy(), ph)
+            np.testing.assert_array_equal(p.status.cpu().numpy(), st)
+
+
 def test_ensemble_graph_replays_equal_eager(lib):
     from turbulence_tracing_amd.ensemble import Ensemble
     ens = Ensemble(small_members(), pix=64)
@@ -120,8 +156,8 @@ def test_full_ensemble_batch_equals_per_member(lib):
     """The ensemble workload at its real size (the seeded draw of 48 members,
     128-384^3, 2e5-2e6 rays each, ~51 M rays resident): the one persistent
     batched launch gives every member's exit rays and phases bit for bit as
-    one tt_trace_rk4 per member, and nearly every ray leaves finite (a ray
-    that turns back is NaN by design)."""
+    one trace per member (the z-march, then the rays it hands over traced in
+    time), and nearly every ray leaves finite."""
     from turbulence_tracing_amd.ensemble import Ensemble, draw_members
     members = draw_members(48, 0)
     assert len(members) == 48 and sum(m.n_rays for m in members) > 40_000_000

@@ -66,7 +66,7 @@ class GemmArgs(C.Structure):
 class BatchItem(C.Structure):
     _fields_ = [("field4", C.c_void_p), ("grid", Grid), ("s0", C.c_void_p), ("ld_s0", C.c_int64),
                 ("n", C.c_int64), ("order", C.c_void_p), ("rf", C.c_void_p), ("ld_rf", C.c_int64),
-                ("phase", C.c_void_p)]
+                ("phase", C.c_void_p), ("status", C.c_void_p), ("cT", C.c_double)]
 
 
 class BatchInfo(C.Structure):

@@ -825,9 +825,8 @@ __constant__ double kDPA[6][6] = {
 // fifth-order minus fourth-order weights, over the seven stages
 __constant__ double kDPE[7] = {71.0 / 57600, 0, -71.0 / 16695, 71.0 / 1920, -17253.0 / 339200, 22.0 / 525, -1.0 / 40};
 
-__global__ __launch_bounds__(256) void trace_time_kernel(TraceArgs a, int32_t only) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.n) return;
+// ray t of a (the order's t-th) if its status is `only` (or a has none)
+__device__ void trace_time_ray(const TraceArgs &a, int64_t t, int32_t only) {
     const int64_t r = a.order ? (int64_t)a.order[t] : t;
     if (a.status && a.status[r] != only) return;
     const double *s = a.s0 + r;
@@ -907,6 +906,22 @@ __global__ __launch_bounds__(256) void trace_time_kernel(TraceArgs a, int32_t on
         for (int i = 0; i < 6; ++i) sf[i * a.ld_sf] = bad ? nanv : (i < 3 ? y[i] : tt::kC * y[i]);
     }
     if (a.status) a.status[r] = bad ? 2 : 3;
+}
+
+__global__ __launch_bounds__(256) void trace_time_kernel(TraceArgs a, int32_t only) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < a.n) trace_time_ray(a, t, only);
+}
+
+// the rays of a batch the z-march handed over (status 1): one block per chunk
+// of the batch's chunk table, items without a status array skipped
+__global__ __launch_bounds__(256) void trace_time_batch_kernel(const TraceArgs *__restrict__ items,
+                                                               const int32_t *__restrict__ chunk_item,
+                                                               const int64_t *__restrict__ chunk_start, int chunk) {
+    const TraceArgs &a = items[chunk_item[blockIdx.x]];
+    if (!a.status) return;
+    const int64_t s = chunk_start[blockIdx.x], e = min(s + (int64_t)chunk, a.n);
+    for (int64_t t = s + threadIdx.x; t < e; t += blockDim.x) trace_time_ray(a, t, 1);
 }
 
 // trilinear samples of the node field at arbitrary points (metres)
@@ -1118,12 +1133,13 @@ extern "C" int tt_batch_prepare(const tt_batch_item *items, int32_t nitems, doub
         TT_REQUIRE(it.field4 && it.s0 && it.rf && it.n >= 0, "tt_batch_prepare: item %d: null pointer or n < 0", i);
         TT_REQUIRE(it.ld_s0 >= it.n && it.ld_rf >= it.n, "tt_batch_prepare: item %d: leading dimension < n", i);
         TT_REQUIRE((int64_t)(it.grid.nz - 1) * m < (1 << 23), "tt_batch_prepare: item %d: too many steps", i);
+        TT_REQUIRE(!it.status || it.cT > 0.0, "tt_batch_prepare: item %d: cT must be > 0 with a status array", i);
         TraceArgs a{};
         if (int rc = fill_grid_args(&it.grid, a)) return rc;
         a.field = reinterpret_cast<const float4 *>(it.field4);
         a.m = m;
         a.h = 1.0f / (float)m;
-        a.cT = 0.0;
+        a.cT = it.status ? it.cT : 0.0;
         a.s0 = it.s0;
         a.ld_s0 = it.ld_s0;
         a.n = it.n;
@@ -1133,7 +1149,7 @@ extern "C" int tt_batch_prepare(const tt_batch_item *items, int32_t nitems, doub
         a.phase = it.phase;
         a.sf = nullptr;
         a.ld_sf = 0;
-        a.status = nullptr;
+        a.status = it.status;
         args[(size_t)i] = a;
     }
     // chunks of the items with the longest rays (most z-steps) first
@@ -1194,5 +1210,9 @@ extern "C" int tt_batch_trace(void *workspace, const tt_batch_info *info, const 
     const int64_t *cstart = reinterpret_cast<const int64_t *>(w + L.cstart);
     hipLaunchKernelGGL(kern, grid, dim3(tune->block), 0, st, hdr, items, citem, cstart, info->nchunks, info->chunk);
     TT_LAUNCH_CHECK("tt_batch_trace launch");
+    // the rays handed over (items with a status array), after the whole z-march
+    hipLaunchKernelGGL(trace_time_batch_kernel, dim3((unsigned)info->nchunks), dim3(256), 0, st, items, citem, cstart,
+                       info->chunk);
+    TT_LAUNCH_CHECK("tt_batch_trace time-pass launch");
     return TT_OK;
 }

@@ -79,8 +79,9 @@ class Ensemble:
         arr = (_lib.BatchItem * len(self.pipes))()
         for i, p in enumerate(self.pipes):
             N = p.cfg.n_rays
+            # with the status array, the rays the z-march hands over are traced in time in the same call
             arr[i] = _lib.BatchItem(p.field4.data_ptr(), p.grid, p.s0.data_ptr(), N, N, None, p.rf.data_ptr(), N,
-                                    p.phase.data_ptr())
+                                    p.phase.data_ptr(), p.status.data_ptr(), p.opts.cT)
         return arr
 
     def prepare(self, chunk=None):
