@@ -181,13 +181,16 @@ int tt_trace_time(const float *field4, const tt_grid *grid, const tt_trace_opts 
 /* tt_trace_rk4 over a ray list whose length is in device memory: the rays
  * order[0 .. *n_dev) are traced (n_dev: device int32, read by the kernel;
  * n_cap >= *n_dev bounds the launch), with no host synchronisation between
- * the producer of the list and the trace.  No sf / status outputs.  No
- * reference counterpart as a call: it serves the surrogate's fallback, which
- * re-traces the rays tt_surrogate_apply flags (the full solve, PT:312-380,
- * on those rays). */
+ * the producer of the list and the trace.  No sf output; status (or NULL) as
+ * in tt_trace_rk4, written for the listed rays only, so that tt_trace_time
+ * over all rays with only_status 1 then traces the listed rays the z-march
+ * handed over (the other rays' entries must not be 1).  No reference
+ * counterpart as a call: it serves the surrogate's fallback, which re-traces
+ * the rays tt_surrogate_apply flags (the full solve, PT:312-380, on those
+ * rays). */
 int tt_trace_rk4_counted(const float *field4, const tt_grid *grid, const tt_trace_opts *opts, const double *s0,
                          int64_t ld_s0, int64_t n_cap, const int32_t *order, const int32_t *n_dev, double *rf,
-                         int64_t ld_rf, double *phase, void *stream);
+                         int64_t ld_rf, double *phase, int32_t *status, void *stream);
 
 /* Ensembles: the rays of many realisations (uneven grids and ray counts)
  * traced in ONE persistent launch.  No reference counterpart as a call: it

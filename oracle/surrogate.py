@@ -13,7 +13,7 @@ reference function: the reference has no surrogate.  Pins:
 """
 import numpy as np
 
-from .tracer import C, trilinear
+from .tracer import C, W_HANDOVER, trilinear
 
 NSEG = 16
 
@@ -83,7 +83,13 @@ def path_estimate(grid, s0, k=1, moments=True):
     y, the x and y column changes between steps, the summed second-order
     proxies q_x = (dGx/dx Gx + dGx/dy Gy) / W^2 and q_y likewise, and the
     path length in cells.  ``moments=False`` drops both first-moment
-    corrections (a check that they matter, for the tests)."""
+    corrections (a check that they matter, for the tests).
+
+    A ray the full trace would hand over to the time integrator, judged on
+    the estimate's own path (W <= W_HANDOVER at launch or at the exit, or x / y
+    inside the grid at the exit but not at the entry or the reverse), gets a
+    NaN base, so the surrogate flags it for the fallback (path.hip
+    path_kernel)."""
     nx, ny, nz = grid.shape
     x0, y0, z0 = grid.origin
     dx, dy, dz = grid.spacing
@@ -183,6 +189,11 @@ def path_estimate(grid, s0, k=1, moments=True):
     px, py, U, V, W, ph = st["px"], st["py"], st["U"], st["V"], st["W"], st["ph"]
     L = ncell - Zs
     base = np.stack([x0 + px * dx, np.arctan(U / W), y0 + py * dy, np.arctan(V / W), ph])
+    fx1, fy1 = nx - 1.0, ny - 1.0
+    in0 = (px0 >= 0) & (px0 <= fx1) & (py0 >= 0) & (py0 <= fy1)
+    in1 = (px >= 0) & (px <= fx1) & (py >= 0) & (py <= fy1)
+    handover = ~(W0 - 1.0 > W_HANDOVER - 1.0) | ~(W - 1.0 > W_HANDOVER - 1.0) | (in0 != in1)
+    base[:, handover] = np.nan
     glob = np.stack([U - U0, V - V0, W - W0, px - (px0 + sx * U0 / W0 * L), py - (py0 + sy * V0 / W0 * L), ph,
                      U0, V0, W0 - 1.0, px0 - np.floor(px0), py0 - np.floor(py0), st["cx"], st["cy"], st["qx"],
                      st["qy"], L], axis=1)

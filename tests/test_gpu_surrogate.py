@@ -182,7 +182,9 @@ def test_path_k1_node_field_matches_block_fields(lib, inside):
     evaluates each node plane's bilinear form; it must agree with the k = 1
     block-field path to float rounding, on a wide beam
     whose rays change column many times (a ray within rounding of a column
-    boundary may count one column change more or less)."""
+    boundary may count one column change more or less).  The beam's steep
+    rays and those that leave through a side face are handed over (NaN
+    estimate) by both paths alike."""
     import torch
     from turbulence_tracing_amd.particle_tracker import ElectronCube
     from turbulence_tracing_amd.surrogate import block_fields, path_estimate
@@ -197,8 +199,11 @@ def test_path_k1_node_field_matches_block_fields(lib, inside):
     b_blk, f_blk = path_estimate(cube.field4, cube.grid, s0, k=1, blocks=blocks)
     b_node, f_node = path_estimate(cube.field4, cube.grid, s0, k=1)
     bb, bn = b_blk.cpu().numpy(), b_node.cpu().numpy()
-    fin = np.isfinite(bb).all(axis=0)
-    assert fin.all() and np.isfinite(bn).all()
+    fb_ok, fn_ok = np.isfinite(bb).all(axis=0), np.isfinite(bn).all(axis=0)
+    assert (fb_ok != fn_ok).sum() <= 10  # a ray within rounding of a side face
+    fin = fb_ok & fn_ok
+    assert 0.5 < fin.mean() < 1.0
+    bb, bn = bb[:, fin], bn[:, fin]
     dx = x[1] - x[0]
     assert np.abs(bn[[0, 2]] - bb[[0, 2]]).max() < 1e-5 * dx
     assert np.abs(bn[[1, 3]] - bb[[1, 3]]).max() < 1e-5 * np.abs(bb[[1, 3]]).max()
@@ -404,11 +409,59 @@ def test_counted_trace_and_wave_ordered_list(lib):
     got_rf = torch.full((4, n), 7.0, dtype=torch.float64, device=DEV)
     got_ph = torch.full((n,), 7.0, dtype=torch.float64, device=DEV)
     lib.tt_trace_rk4_counted(cube.field4.data_ptr(), ctypes.byref(cube.grid), ctypes.byref(opts), s0.data_ptr(), n,
-                             n, lst.data_ptr(), cnt.data_ptr(), got_rf.data_ptr(), n, got_ph.data_ptr(),
+                             n, lst.data_ptr(), cnt.data_ptr(), got_rf.data_ptr(), n, got_ph.data_ptr(), None,
                              _lib.stream_ptr())
     torch.cuda.synchronize()
     assert torch.equal(got_rf, want_rf) and torch.equal(got_ph, want_ph)
     assert bool((got_ph[torch.as_tensor(~fl, device=DEV)] == 7.0).all())
+
+
+def test_surrogate_hands_over_like_the_plain_trace(lib):
+    """Rays the full trace hands over to the time integrator (every 5th ray
+    launched at 0.3 rad; every 7th other one started half a cell inside the
+    +x face and heading out) get a NaN estimate, so the surrogate flags them
+    whatever its network says, and its fallback re-traces them as the plain
+    trace does (z-march, then the time integrator): rf and phase bit for bit,
+    status 3."""
+    from turbulence_tracing_amd import _lib
+    from turbulence_tracing_amd.particle_tracker import ElectronCube, c
+    from turbulence_tracing_amd.surrogate import MLP, Norm, Surrogate
+    x, ne = small_cube(n=33)
+    cube = ElectronCube(x, x.copy(), x.copy())
+    cube.external_ne(ne)
+    cube.calc_dndr(lwl=1064e-9)
+    n = 3000
+    s0 = cube.init_beam(n, 2e-3, 50e-6, seed=13).clone()
+    steep = torch.zeros(n, dtype=torch.bool, device=DEV)
+    steep[::5] = True
+    side = torch.zeros(n, dtype=torch.bool, device=DEV)
+    side[3::7] = True
+    side &= ~steep
+    dx = float(x[1] - x[0])
+    s0[3, steep] = 0.3 * c
+    s0[4, steep] = 0.0
+    s0[5, steep] = float(np.sqrt(1.0 - 0.09)) * c
+    s0[0, side] = float(x.max()) - 0.5 * dx
+    s0[3, side] = 0.05 * c
+    s0[4, side] = 0.0
+    s0[5, side] = float(np.sqrt(1.0 - 0.0025)) * c
+    opts = _lib.TraceOpts(0.5, float(np.sqrt(8.0) * x.max()), 0)
+    want_rf = torch.empty((4, n), dtype=torch.float64, device=DEV)
+    want_ph = torch.empty(n, dtype=torch.float64, device=DEV)
+    want_st = torch.empty(n, dtype=torch.int32, device=DEV)
+    args = (cube.field4.data_ptr(), C.byref(cube.grid), C.byref(opts), s0.data_ptr(), n, n, None,
+            want_rf.data_ptr(), n, want_ph.data_ptr(), None, n, want_st.data_ptr())
+    assert lib.tt_trace_rk4(*args, _lib.stream_ptr()) == 0
+    assert lib.tt_trace_time(*args, 1, _lib.stream_ptr()) == 0
+    sur = Surrogate(Norm(np.zeros(48, np.float32), np.ones(48, np.float32), 1e-6, 1e-6, 1e-3), mlp=MLP(seed=2), k=4)
+    rf, ph, fl = sur.trace(cube.field4, cube.grid, s0, t_pos=1.0, t_ang=1.0)  # bounds flag nothing by themselves
+    torch.cuda.synchronize()
+    sel = steep | side
+    assert bool((want_st[sel] == 3).all()), "the plain trace hands these rays over"
+    assert bool(fl[sel].bool().all())
+    assert torch.equal(rf[:, sel], want_rf[:, sel]) and torch.equal(ph[sel], want_ph[sel])
+    assert bool((sur.status[sel] == 3).all())
+    assert bool((sur.status[~fl.bool()] == 0).all())
 
 
 def test_surrogate_trace_graph_capture_replays(lib):

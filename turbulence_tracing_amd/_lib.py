@@ -94,7 +94,7 @@ _SIGNATURES = {
     "tt_plane_forms": [P, C.POINTER(Grid), P, P],
     "tt_trace_rk4": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, I64, P, P, I64, P, P],
     "tt_trace_time": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, I64, P, P, I64, P, I32, P],
-    "tt_trace_rk4_counted": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, P, I64, P, P],
+    "tt_trace_rk4_counted": [P, C.POINTER(Grid), C.POINTER(TraceOpts), P, I64, I64, P, P, P, I64, P, P, P],
     "tt_sample_field": [P, C.POINTER(Grid), P, I64, I64, P, I64, P],
     "tt_beam_sample": [I64, D, D, D, I32, U64, U64, P, I64, P],
     "tt_ray_sort_scratch_bytes": [C.POINTER(Grid), I32, SIZE_P],

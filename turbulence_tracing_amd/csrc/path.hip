@@ -275,7 +275,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NODE ? 5 : 
     float *f = a.feat + r;  // feature-major: feature q of this ray at f[q * ldf]
     double *b = a.base + r;
     const double nanv = __longlong_as_double(0x7ff8000000000000ll);
-    if (!(W0 > 0.0) || isnan(x) || isnan(y) || isnan(z) || isnan(U0) || isnan(V0)) {
+    // a ray the full trace hands over at launch (W0 <= 1 + kWdMin, which
+    // includes W0 <= 0) is out of the estimate's reach: NaN, so the surrogate
+    // flags it for the fallback
+    if (!(W0 - 1.0 > (double)tt::kWdMin) || isnan(x) || isnan(y) || isnan(z) || isnan(U0) || isnan(V0)) {
         for (int c = 0; c < 5; ++c) b[c * a.ldb] = nanv;
         for (int c = 0; c < 48; ++c) f[c * a.ldf] = 0.f;
         return;
@@ -371,11 +374,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NODE ? 5 : 
     const double ph = S.ph;
     const double Uf = U, Vf = V, Wf = 1.0 + (double)Wd;
     const double pxe = px0 + (double)dpx, pye = py0 + (double)dpy;
-    b[0] = a.x0 + pxe * a.dx;
-    b[a.ldb] = small_atan(Uf / Wf);
-    b[2 * a.ldb] = a.y0 + pye * a.dy;
-    b[3 * a.ldb] = small_atan(Vf / Wf);
-    b[4 * a.ldb] = ph;
+    // the full trace's other hand-overs, judged on the estimate's own path:
+    // W at or below 1 + kWdMin at the exit, or a side face of the grid crossed
+    // (x / y inside the grid at the exit but not at the entry, or the reverse)
+    const double fx1 = (double)(a.nx - 1), fy1 = (double)(a.ny - 1);
+    const bool in0 = px0 >= 0.0 && px0 <= fx1 && py0 >= 0.0 && py0 <= fy1;
+    const bool in1 = pxe >= 0.0 && pxe <= fx1 && pye >= 0.0 && pye <= fy1;
+    const bool handover = !(Wd > tt::kWdMin) || in0 != in1;
+    b[0] = handover ? nanv : a.x0 + pxe * a.dx;
+    b[a.ldb] = handover ? nanv : small_atan(Uf / Wf);
+    b[2 * a.ldb] = handover ? nanv : a.y0 + pye * a.dy;
+    b[3 * a.ldb] = handover ? nanv : small_atan(Vf / Wf);
+    b[4 * a.ldb] = handover ? nanv : ph;
     const double L = (double)ncell - Zs;
     float *gf = f + 2 * kSeg * a.ldf;
     const int64_t L1 = a.ldf;

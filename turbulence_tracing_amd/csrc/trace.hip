@@ -184,7 +184,9 @@ __device__ __forceinline__ float phi_at(const TraceArgs &a, float pxb, float pyb
 // point.  With ne/nc clipped at 1 the field changes W by at most ~0.3 per
 // step, so a ray cannot pass from W > 0.97 to a turning point within one
 // step unnoticed.  The test costs nothing beyond the W > 0 test it replaces.
-constexpr float kWdMin = -0.03f;  // W - 1 at the hand-over
+// The path estimate (path.hip) flags the same rays for the surrogate's
+// fallback.
+using tt::kWdMin;  // W - 1 at the hand-over
 
 __device__ __forceinline__ bool lateral_inside(const TraceArgs &a, float pxb, float pyb, const State &s) {
     const float px = fmaf(a.sx, s.qx, pxb), py = fmaf(a.sy, s.qy, pyb);
@@ -1080,9 +1082,10 @@ extern "C" int tt_trace_time(const float *field4, const tt_grid *grid, const tt_
 
 extern "C" int tt_trace_rk4_counted(const float *field4, const tt_grid *grid, const tt_trace_opts *opts,
                                     const double *s0, int64_t ld_s0, int64_t n_cap, const int32_t *order,
-                                    const int32_t *n_dev, double *rf, int64_t ld_rf, double *phase, void *stream) {
+                                    const int32_t *n_dev, double *rf, int64_t ld_rf, double *phase, int32_t *status,
+                                    void *stream) {
     TT_REQUIRE(n_cap == 0 || (order && n_dev), "tt_trace_rk4_counted: order and n_dev are required");
-    return trace_launch(field4, grid, opts, s0, ld_s0, n_cap, order, n_dev, rf, ld_rf, phase, nullptr, ld_rf, nullptr,
+    return trace_launch(field4, grid, opts, s0, ld_s0, n_cap, order, n_dev, rf, ld_rf, phase, nullptr, ld_rf, status,
                         stream);
 }
 

@@ -87,4 +87,8 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * q + min(x, r) + b / 8;
 }
 
+// W - 1 (W = vz / c) at or below which the z-march hands a ray over to the
+// time integrator (the rule is stated with the tracer)
+constexpr float kWdMin = -0.03f;
+
 }  // namespace tt

@@ -449,13 +449,19 @@ class Surrogate:
             self._streams = (torch.cuda.Stream(device=self.device), torch.cuda.Stream(device=self.device))
         return self._streams
 
-    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, blocks=None, timings=None, chunks=1):
+    def trace(self, field4, grid, s0, h=0.5, t_pos=1e-6, t_ang=1e-6, blocks=None, timings=None, chunks=1,
+              cT=None):
         """Surrogate + fallback trace of rays s0 (6 x n, device): the block
-        path estimate, the network's correction, and the full RK4 tracer on
-        the rays the network flags (predicted position error above t_pos
-        metres or angle error above t_ang radians, or a non-finite
-        estimate).  Returns rf (4 x n), phase (n) and the flag per ray
-        (1 = re-traced).
+        path estimate, the network's correction, and the full tracer on the
+        rays the network flags (predicted position error above t_pos metres
+        or angle error above t_ang radians, or a non-finite estimate, which
+        includes every ray the full trace would hand over to the time
+        integrator).  The fallback is the z-march, then the time integrator
+        on the flagged rays it hands over, over the light-travel budget cT
+        (metres; default sqrt(8) times the grid's largest z, as solve's time
+        span, PT:312-331).  Returns rf (4 x n), phase (n) and the flag per ray
+        (1 = re-traced); ``self.status`` holds the re-traced rays' status as
+        tt_trace_rk4 / tt_trace_time leave it (0 for the others).
 
         With ``chunks`` > 1 the rays are cut into that many batches: the path
         estimate of batch i + 1 runs on one stream while the network and the
@@ -522,11 +528,18 @@ class Surrogate:
             t1 = torch.cuda.Event(enable_timing=True)
             t1.record(main)
         # the flagged rays, their number read by the kernel from cnt: no host
-        # synchronisation, so waves of rays and graph capture stay unblocked
-        opts = _lib.TraceOpts(h, 0.0, 0)
+        # synchronisation, so waves of rays and graph capture stay unblocked;
+        # then those of them the z-march handed over (status 1), in time
+        if cT is None:
+            cT = float(np.sqrt(8.0) * max(grid.z0 + (grid.nz - 1) * grid.dz, 0.5 * (grid.nz - 1) * grid.dz))
+        opts = _lib.TraceOpts(h, cT, 0)
+        status = torch.zeros(n, dtype=torch.int32, device=dev)
         self.lib.tt_trace_rk4_counted(_p(field4), C.byref(grid), C.byref(opts), _p(s0), s0.stride(0), n, _p(lst),
-                                      _p(cnt), _p(rf), n, _p(phase), _lib.stream_ptr())
+                                      _p(cnt), _p(rf), n, _p(phase), _p(status), _lib.stream_ptr())
+        self.lib.tt_trace_time(_p(field4), C.byref(grid), C.byref(opts), _p(s0), s0.stride(0), n, None, _p(rf), n,
+                               _p(phase), None, n, _p(status), 1, _lib.stream_ptr())
         self.n_fallback = cnt  # device int32: rays re-traced by the last call
+        self.status = status
         if timings is not None:
             t2 = torch.cuda.Event(enable_timing=True)
             t2.record(main)
