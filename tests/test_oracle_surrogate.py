@@ -96,6 +96,31 @@ def test_path_null_and_uniform_exact():
     np.testing.assert_allclose(base[4], T.wavenumber(1064e-9) * (np.sqrt(1 - r) - 1) * 10e-3, rtol=1e-12)
 
 
+def test_path_hands_over_the_rays_the_zmarch_hands_over():
+    """On a null field the estimate's path is the z-march's straight line, so
+    the rays it gives a NaN estimate (for the surrogate's fallback) are
+    exactly those the oracle z-march hands over (status 1): steep at launch
+    (W <= 0.97), or crossing a side face of the grid; the others are
+    finite and equal the z-march's exits."""
+    n = 9
+    c = cube_of(np.zeros((n, n, n)), n)
+    C = T.C
+    th = np.array([0.0, 0.1, 0.2, 0.23, 0.26, 0.3, 0.05, 0.05, -0.05, 0.2])  # slope angle in x
+    x0 = np.array([0.0, 0.0, -1e-3, -2e-3, 0.0, 0.0, 4.9e-3, 4.0e-3, -4.9e-3, -4.9e-3])
+    s0 = np.zeros((6, th.size))
+    s0[0] = x0
+    s0[2] = -5e-3
+    s0[3] = C * np.sin(th)
+    s0[5] = C * np.cos(th)
+    base, _ = S.path_estimate(T.NodeGrid(c), s0)
+    rf, _, _, st = T.zmarch_rk4(T.NodeGrid(c), s0)
+    nan = ~np.isfinite(base).all(axis=0)
+    # W = cos(th) <= 0.97 for |th| >= 0.2443; 4.9 mm + 10 mm * tan(0.05) leaves the +-5 mm grid
+    np.testing.assert_array_equal(nan, [False, False, False, False, True, True, True, False, True, False])
+    np.testing.assert_array_equal(nan, st == 1)
+    np.testing.assert_allclose(base[:4, ~nan], rf[:, ~nan], atol=1e-15)
+
+
 def test_path_constant_gradient_matches_parabola():
     """A constant transverse gradient: the mid-plane kick integrates the
     parabolic ray exactly (positions), and the angle to the order of W's
