@@ -377,10 +377,12 @@ __device__ __forceinline__ float2 philox_pair(int64_t lin, uint32_t k0, uint32_t
     const tt::u32x4 r =
         tt::philox4x32_10(tt::u32x4{(uint32_t)lin, (uint32_t)((uint64_t)lin >> 32), tt::kStreamField, 0u}, k0, k1);
     const float u1 = tt::u01f(r.x), u2 = tt::u01f(r.y);
-    const float rad = sqrtf(-2.0f * logf(u1));
-    float s, c;
-    sincospif(2.0f * u2, &s, &c);
-    return make_float2(rad * c, rad * s);
+    // Box-Muller on the hardware log2, sqrt and sin / cos (v_sin / v_cos take
+    // revolutions, so sin(2 pi u2) is v_sin(u2)); u1, u2 lie in (0, 1) and are
+    // never denormal.  Within ~1e-6 of the correctly rounded forms, which cost
+    // 15% of the field's time
+    const float rad = __builtin_amdgcn_sqrtf(-2.0f * 0.69314718055994531f * __builtin_amdgcn_logf(u1));
+    return make_float2(rad * __builtin_amdgcn_cosf(u2), rad * __builtin_amdgcn_sinf(u2));
 }
 
 #pragma clang fp contract(off)
@@ -388,7 +390,7 @@ __device__ float sqrt_psd_native(const PassArgs &a, int jx, int jy, int jz) {
     const double fx = fftfreq(jx, a.nx), fy = fftfreq(jy, a.ny), fz = fftfreq(jz, a.nz) / a.aniso;
     const double Ka = sqrt((fx * fx + fy * fy) + fz * fz);
     if (!(Ka >= a.kmin && Ka <= a.kmax && Ka > 0.0)) return 0.0f;
-    return a.sqrt_amp * powf((float)Ka, a.half_beta);
+    return a.sqrt_amp * __builtin_amdgcn_exp2f(a.half_beta * __builtin_amdgcn_logf((float)Ka));  // Ka^(beta/2)
 }
 #pragma clang fp contract(on)
 
