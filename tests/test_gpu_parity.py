@@ -100,7 +100,7 @@ def small_cube(n=33, seed=2, eps=0.1, ne0=1e24, L=10e-3):
 
 
 def test_grad_stencil_matches_calc_dndr(lib, golden):
-    from turbulence_tracing_amd.particle_tracker import ElectronCube, c
+    from turbulence_tracing_amd.particle_tracker import ElectronCube
     x = golden["cube_x"]
     cube = ElectronCube(x, x.copy(), x.copy())
     cube.external_ne(golden["cube_ne"])

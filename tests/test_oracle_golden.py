@@ -1,7 +1,5 @@
 """The FP64 oracle against the vectors written from the reference itself
 (tests/golden/make_golden.py) and against closed forms."""
-import io
-
 import numpy as np
 import pytest
 
